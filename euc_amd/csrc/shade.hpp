@@ -1,0 +1,194 @@
+// shade.hpp -- device-side scalar semantics and the fixed shader set.
+//
+// Every function here is evaluated with -ffp-contract=off and IEEE f32
+// division/sqrt, so each operation rounds once in the order written.  The
+// order follows the reference's expressions (triangles.rs, math.rs,
+// sampler/linear.rs) and this project's fixed-shader definitions.
+#pragma once
+#include "internal.hpp"
+
+namespace euchip {
+
+#define EUCHIP_DEV __device__ __forceinline__
+
+// f32::min / f32::max (the non-NaN operand wins).
+EUCHIP_DEV float rmin(float a, float b) { return fminf(a, b); }
+EUCHIP_DEV float rmax(float a, float b) { return fmaxf(a, b); }
+// f32::fract (self - self.trunc()).
+EUCHIP_DEV float rfract(float x) { return x - truncf(x); }
+
+// `(f as usize).clamp(lo, hi)` for hi < 2^24: saturating, NaN -> 0.
+EUCHIP_DEV uint32_t sat_clamp(float f, uint32_t lo, uint32_t hi) {
+    uint32_t v;
+    if (!(f > 0.0f)) v = 0;
+    else if (f >= (float)hi) v = hi;
+    else v = (uint32_t)f;
+    return v < lo ? lo : v;
+}
+
+// `f as u8`: saturating truncation, NaN -> 0.
+EUCHIP_DEV uint32_t sat_u8(float f) {
+    if (!(f > 0.0f)) return 0;
+    if (f >= 255.0f) return 255;
+    return (uint32_t)f;
+}
+
+EUCHIP_DEV void mat4_vec4(const float *m, const float v[4], float r[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        r[i] = ((m[4 * i + 0] * v[0] + m[4 * i + 1] * v[1]) + m[4 * i + 2] * v[2]) + m[4 * i + 3] * v[3];
+}
+
+EUCHIP_DEV void normalize3(const float v[3], float r[3]) {
+    float l2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    float s = 1.0f / sqrtf(l2);
+    r[0] = v[0] * s;
+    r[1] = v[1] * s;
+    r[2] = v[2] * s;
+}
+
+EUCHIP_DEV float dot3s(const float a[3], const float b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+EUCHIP_DEV float pow30(float x) {
+    float x2 = x * x, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
+    float x24 = x16 * x8, x28 = x24 * x4;
+    return x28 * x2;
+}
+
+// Two directional lights: ambient + sum_i I_i * (kd * max(n.l, 0) + ks * max(r.v, 0)^30).
+EUCHIP_DEV void lighting2(const float *u, const float wpos[3], const float nrm[3], float out[4]) {
+    float n[3], vv[3], v[3];
+    normalize3(nrm, n);
+    vv[0] = u[U_CAM + 0] - wpos[0];
+    vv[1] = u[U_CAM + 1] - wpos[1];
+    vv[2] = u[U_CAM + 2] - wpos[2];
+    normalize3(vv, v);
+    float light = u[U_K + 0];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const float *L = u + (i == 0 ? U_L0 : U_L1);
+        float ndl = dot3s(n, L);
+        float diff = rmax(ndl, 0.0f) * u[U_K + 1];
+        float t = ndl * 2.0f;
+        float r[3] = {t * n[0] - L[0], t * n[1] - L[1], t * n[2] - L[2]};
+        float rv = rmax(dot3s(r, v), 0.0f);
+        float spec = pow30(rv) * u[U_K + 2];
+        light = light + (diff + spec) * L[3];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) out[c] = u[U_SURF + c] * light;
+}
+
+// Bilinear sample, Clamped<Linear> (sampler/mod.rs:183-186, sampler/linear.rs:30-66)
+// of an RGBA8 texture mapped to f32 per channel, or of an f32 map (nc = 1).
+template <int NC>
+EUCHIP_DEV void sample_bilinear(const void *data, uint32_t w, uint32_t h, float x, float y, bool clamped,
+                                float *out) {
+    if (clamped) {
+        x = rmin(rmax(x, 0.0f), 1.0f);
+        y = rmin(rmax(y, 0.0f), 1.0f);
+    }
+    float itx = rfract(x) * (float)w;
+    float ity = rfract(y) * (float)h;
+    float tx = truncf(itx), ty = truncf(ity);
+    uint32_t px = !(tx > 0.0f) ? 0u : (tx >= (float)w ? w : (uint32_t)tx);
+    uint32_t py = !(ty > 0.0f) ? 0u : (ty >= (float)h ? h : (uint32_t)ty);
+    float fx = rfract(itx);
+    float fy = rfract(ity);
+    uint32_t p0x = px < w - 1 ? px : w - 1;
+    uint32_t p0y = py < h - 1 ? py : h - 1;
+    uint32_t p1x = px + 1 < w - 1 ? px + 1 : w - 1;
+    uint32_t p1y = py + 1 < h - 1 ? py + 1 : h - 1;
+    if (NC == 1) {
+        const float *t = (const float *)data;
+        float t00 = t[p0x + w * p0y], t10 = t[p1x + w * p0y], t01 = t[p0x + w * p1y], t11 = t[p1x + w * p1y];
+        float t0 = t00 * (1.0f - fy) + t01 * fy;
+        float t1 = t10 * (1.0f - fy) + t11 * fy;
+        out[0] = t0 * (1.0f - fx) + t1 * fx;
+    } else {
+        const uint32_t *t = (const uint32_t *)data;
+        uint32_t q00 = t[p0x + w * p0y], q10 = t[p1x + w * p0y], q01 = t[p0x + w * p1y], q11 = t[p1x + w * p1y];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            float t00 = (float)((q00 >> (8 * c)) & 0xffu), t10 = (float)((q10 >> (8 * c)) & 0xffu);
+            float t01 = (float)((q01 >> (8 * c)) & 0xffu), t11 = (float)((q11 >> (8 * c)) & 0xffu);
+            float t0 = t00 * (1.0f - fy) + t01 * fy;
+            float t1 = t10 * (1.0f - fy) + t11 * fy;
+            out[c] = t0 * (1.0f - fx) + t1 * fx;
+        }
+    }
+}
+
+// Shadow-map lookup of examples/teapot.rs:123-128.
+EUCHIP_DEV bool in_light(const FrameParams &p, const float lvp[3]) {
+    float sx = (lvp[0] * 1.0f) * 0.5f + 0.5f;
+    float sy = (lvp[1] * -1.0f) * 0.5f + 0.5f;
+    float ld;
+    sample_bilinear<1>(p.shadow, p.sh_w, p.sh_h, sx, sy, true, &ld);
+    ld = ld + p.u[U_K + 3];
+    return lvp[2] < ld;
+}
+
+// Pipeline::fragment of the fixed shader set.  vd: VertexData; out: Rgba<f32>
+// (FLAT: colour bits in out[0]).
+EUCHIP_DEV void fragment_shader(const FrameParams &p, const float *vd, float out[4]) {
+    const float *u = p.u;
+    out[0] = out[1] = out[2] = out[3] = 0.0f;
+    switch (p.shader) {
+    case FLAT: out[0] = vd[0]; break;
+    case COLOR:
+    case GOURAUD:
+        out[0] = vd[0], out[1] = vd[1], out[2] = vd[2], out[3] = vd[3];
+        break;
+    case PHONG: lighting2(u, vd, vd + 3, out); break;
+    case TEXTURED: sample_bilinear<4>(p.tex, p.tex_w, p.tex_h, vd[0], vd[1], false, out); break;
+    case PHONG_SHADOW: {
+        float wn[3], cd[3], ld[3], t3[3];
+        normalize3(vd + 3, wn);
+        for (int c = 0; c < 3; c++) t3[c] = u[U_CAM + c] - vd[c];
+        normalize3(t3, cd);
+        for (int c = 0; c < 3; c++) t3[c] = vd[c] - u[U_LPOS + c];
+        normalize3(t3, ld);
+        float nl[3] = {-ld[0], -ld[1], -ld[2]};
+        float diffuse = rmax(dot3s(wn, nl), 0.0f) * u[U_K + 1];
+        float dn = dot3s(nl, wn) * 2.0f;
+        float r[3] = {nl[0] - wn[0] * dn, nl[1] - wn[1] * dn, nl[2] - wn[2] * dn};
+        float ncd[3] = {-cd[0], -cd[1], -cd[2]};
+        float specular = pow30(rmax(dot3s(r, ncd), 0.0f)) * u[U_K + 2];
+        float light = u[U_K + 0] + (in_light(p, vd + 6) ? diffuse + specular : 0.0f);
+        for (int c = 0; c < 4; c++) out[c] = u[U_SURF + c] * light;
+        break;
+    }
+    case TEX_SHADOW: {
+        float n[3], texc[4];
+        normalize3(vd + 3, n);
+        float ndl = rmax(dot3s(n, u + U_L0), 0.0f);
+        float light = u[U_K + 0] + (in_light(p, vd + 8) ? ndl * u[U_K + 1] : 0.0f);
+        sample_bilinear<4>(p.tex, p.tex_w, p.tex_h, vd[6], vd[7], true, texc);
+        float s = light * (1.0f / 255.0f);
+        out[0] = texc[0] * s;
+        out[1] = texc[1] * s;
+        out[2] = texc[2] * s;
+        out[3] = texc[3] * (1.0f / 255.0f);
+        break;
+    }
+    default: break;
+    }
+}
+
+// Pipeline::blend: every fixed shader replaces the old pixel.
+EUCHIP_DEV uint32_t blend_shader(const FrameParams &p, uint32_t old, const float f[4]) {
+    if (p.shader == FLAT) return __float_as_uint(f[0]);
+    if (p.shader == DEPTH_ONLY) return old;
+    uint32_t b[4];
+    if (p.shader == TEXTURED) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) b[c] = sat_u8(f[c]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; c++) b[c] = sat_u8(rmin(rmax(f[c], 0.0f), 1.0f) * 255.0f);
+    }
+    return b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
+}
+
+}  // namespace euchip

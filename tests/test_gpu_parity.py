@@ -1,0 +1,117 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bar: colour and depth targets bit-identical (u32 / f32 bit patterns) and the
+same status, for every fixed shader whose arithmetic is exact IEEE f32 (all of
+them: the fixed shaders use sqrt and division only, which are correctly
+rounded on both sides, and an exact x^30 chain instead of powf).
+"""
+import numpy as np
+import pytest
+
+from euc_amd import scenes as S
+from euc_amd.pipeline import (AaMode, BandPolicy, CoordinateMode, CullMode, DepthMode, PipelineDesc, PixelMode,
+                              Shader, Xform)
+from tests.parity import assert_bit_exact, run_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def flat(depth_mode=DepthMode.LESS_WRITE, cm=CoordinateMode.VULKAN, cull=CullMode.BACK, bands=BandPolicy.PAR,
+         pixel_mode=PixelMode.WRITE, aa=AaMode.NONE):
+    return PipelineDesc.make(Shader.FLAT, Xform.CLIP, pixel_mode, depth_mode, cm, cull, aa, bands)
+
+
+@pytest.fixture(scope="module")
+def plain():
+    return S.plain_scene()
+
+
+@pytest.mark.parametrize("bands", [BandPolicy.PAR, BandPolicy.SEQ])
+def test_plain_640x480(gpu_ctx, plain, bands):
+    """The CPU-runnable workload: 1024 random clip-space triangles, flat, LESS_WRITE."""
+    r = run_pair(gpu_ctx, flat(bands=bands), plain.uniforms, plain.mesh, 640, 480)
+    assert r["frags"] > 100000
+    assert_bit_exact(r, f"plain {bands.name}")
+
+
+@pytest.mark.parametrize("dm", [DepthMode.NONE, DepthMode.LESS_PASS, DepthMode.GREATER_WRITE,
+                                DepthMode.GREATER_PASS, DepthMode(2, True), DepthMode(2, False)])
+def test_plain_depth_modes(gpu_ctx, plain, dm):
+    init = 0.5 if dm.test in (2, 3) else 1.0
+    r = run_pair(gpu_ctx, flat(depth_mode=dm), plain.uniforms, plain.mesh, 640, 480, depth_init=init)
+    assert_bit_exact(r, f"depth {dm}")
+
+
+@pytest.mark.parametrize("cull", [CullMode.NONE, CullMode.FRONT])
+@pytest.mark.parametrize("cm", [CoordinateMode.OPENGL, CoordinateMode.DIRECTX, CoordinateMode.VULKAN.without_z_clip()])
+def test_plain_cull_coords(gpu_ctx, plain, cull, cm):
+    r = run_pair(gpu_ctx, flat(cull=cull, cm=cm), plain.uniforms, plain.mesh, 640, 480)
+    assert_bit_exact(r, f"{cull.name} {cm}")
+
+
+@pytest.mark.parametrize("wh", [(160, 120), (141, 141), (317, 211), (1000, 37), (33, 900), (1, 1), (2048, 64)])
+@pytest.mark.parametrize("bands", [BandPolicy.PAR, BandPolicy.SEQ])
+def test_plain_odd_sizes(gpu_ctx, plain, wh, bands):
+    """Ragged sizes; PAR leaves targets with H / G == 0 untouched (pipeline.rs:330)."""
+    w, h = wh
+    r = run_pair(gpu_ctx, flat(bands=bands), plain.uniforms, plain.mesh, w, h, color_init=0x11223344)
+    assert_bit_exact(r, f"{wh} {bands.name}")
+
+
+def test_plain_loaded_targets(gpu_ctx, plain):
+    """Targets with prior contents (LOAD, not a fused clear)."""
+    rng = np.random.default_rng(7)
+    c0 = rng.integers(0, 2**32, (480, 640), dtype=np.uint64).astype(np.uint32)
+    d0 = rng.random((480, 640), dtype=np.float32)
+    r = run_pair(gpu_ctx, flat(), plain.uniforms, plain.mesh, 640, 480, color_init=c0, depth_init=d0)
+    assert_bit_exact(r, "loaded")
+
+
+@pytest.fixture(scope="module")
+def torus_small():
+    return S.torus_scene(width=640, height=360, nu=64, nv=32)
+
+
+@pytest.mark.parametrize("shader", [Shader.GOURAUD, Shader.PHONG])
+@pytest.mark.parametrize("bands", [BandPolicy.PAR, BandPolicy.SEQ])
+def test_torus_small(gpu_ctx, torus_small, shader, bands):
+    d = PipelineDesc.make(shader, Xform.MVP, depth_mode=DepthMode.LESS_WRITE, band_policy=bands)
+    sc = torus_small
+    r = run_pair(gpu_ctx, d, sc.uniforms, sc.mesh, sc.width, sc.height)
+    assert r["frags"] > 1000
+    assert_bit_exact(r, f"torus {shader.name} {bands.name}")
+
+
+@pytest.mark.parametrize("level", [1, 2, 3])
+@pytest.mark.parametrize("shader", [Shader.GOURAUD, Shader.PHONG])
+def test_torus_msaa(gpu_ctx, torus_small, level, shader):
+    d = PipelineDesc.make(shader, Xform.MVP, depth_mode=DepthMode.LESS_WRITE, aa_mode=AaMode.Msaa(level))
+    sc = torus_small
+    r = run_pair(gpu_ctx, d, sc.uniforms, sc.mesh, sc.width, sc.height)
+    assert_bit_exact(r, f"msaa {level} {shader.name}")
+
+
+@pytest.fixture(scope="module")
+def torus_1080p():
+    return S.torus_scene()
+
+
+@pytest.mark.parametrize("shader", [Shader.GOURAUD, Shader.PHONG])
+def test_torus_1080p(gpu_ctx, torus_1080p, shader):
+    """The metric workload at full size: 4 x 65,536-triangle tori at 1920x1080."""
+    sc = torus_1080p
+    d = PipelineDesc.make(shader, Xform.MVP, depth_mode=DepthMode.LESS_WRITE)
+    r = run_pair(gpu_ctx, d, sc.uniforms, sc.mesh, sc.width, sc.height)
+    assert r["frags"] > 400000
+    assert_bit_exact(r, f"torus1080 {shader.name}")
+
+
+def test_color_vertex_triangle(gpu_ctx):
+    """examples/triangle.rs: one Gouraud triangle with vertex colours, Empty depth."""
+    pos = np.array([[-1, -1, 0, 1], [1, -1, 0, 1], [0, 1, 0, 1]], np.float32)
+    col = np.array([[1, 0, 0, 1], [0, 1, 0, 1], [0, 0, 1, 1]], np.float32)
+    mesh = S.Mesh(pos=pos, col=col, count=3)
+    for cull in (CullMode.BACK, CullMode.NONE):
+        d = PipelineDesc.make(Shader.COLOR, Xform.CLIP, cull=cull)
+        r = run_pair(gpu_ctx, d, S.uniforms(), mesh, 640, 480, depth=False)
+        assert_bit_exact(r, f"triangle.rs cull={cull.name}")
