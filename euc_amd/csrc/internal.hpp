@@ -6,8 +6,10 @@
 namespace euchip {
 
 constexpr int kUniforms = 64;
-constexpr int kTileW = 8;   // raster tile = one wave64, one pixel per lane
+constexpr int kTileW = 32;  // raster tile: 32 x 8 pixels, one pixel per thread of a 256-thread workgroup
 constexpr int kTileH = 8;
+constexpr int kTileShiftX = 5;
+constexpr int kTileShiftY = 3;
 constexpr int kMaxVD = 12;  // VertexData floats of the widest fixed shader
 
 // Uniform block offsets (same layout as include/euchip.h documents).
@@ -16,36 +18,40 @@ constexpr int U_VP = 0, U_L0 = 16, U_L1 = 20, U_CAM = 24, U_SURF = 28, U_K = 32,
 enum Shader : uint32_t { FLAT = 0, COLOR = 1, GOURAUD = 2, PHONG = 3, DEPTH_ONLY = 4, TEXTURED = 5, PHONG_SHADOW = 6, TEX_SHADOW = 7 };
 enum DepthTest : uint32_t { DT_NONE = 0, DT_LESS = 1, DT_EQUAL = 2, DT_GREATER = 3 };
 
-inline __host__ __device__ uint32_t vd_count(uint32_t shader) {
-    switch (shader) {
-    case FLAT: return 1;
-    case COLOR:
-    case GOURAUD: return 4;
-    case PHONG: return 6;
-    case DEPTH_ONLY: return 0;
-    case TEXTURED: return 2;
-    case PHONG_SHADOW: return 9;
-    case TEX_SHADOW: return 11;
-    }
-    return 0;
+constexpr __host__ __device__ uint32_t vd_count(uint32_t shader) {
+    return shader == FLAT ? 1
+         : (shader == COLOR || shader == GOURAUD) ? 4
+         : shader == PHONG ? 6
+         : shader == DEPTH_ONLY ? 0
+         : shader == TEXTURED ? 2
+         : shader == PHONG_SHADOW ? 9
+         : shader == TEX_SHADOW ? 11 : 0;
 }
 
-// One surviving triangle after setup (triangles.rs:55-171), 128 bytes so the
-// raster kernel fetches it with two wave-uniform scalar loads.
+// One surviving triangle after setup (triangles.rs:55-171), stored at its
+// primitive index; 128 bytes = 8 dwordx4.
 struct alignas(16) TriRec {
     float o[3], dx[3], dy[3];   // w_hom_origin, w_hom_dx, w_hom_dy
     float hz[3];                // verts_hom[i][2] after flip/reversal
     float vby[6];               // verts_by_y (screen x, y) x 3
     uint32_t xmin, xmax;        // bounds_clamped x, clamped to [0, W]
     uint32_t ymin, ymax;        // bounds_clamped y, clamped to [0, H]
-    uint32_t prim;              // triangle index in submission order
     uint32_t flags;             // bit 0: all three vertices pass the z clip
-    uint32_t sid[3];            // shaded-vertex ids (post-reversal order)
-    uint32_t pad[5];
+    uint32_t vi[3];             // vertex index of each (post-reversal) vertex
+    uint32_t inst[3];           // instance of each (post-reversal) vertex
+    uint32_t pad[3];
 };
 static_assert(sizeof(TriRec) == 128, "TriRec must stay 128 bytes");
 
 enum FrameFlags : uint32_t { FLAG_BIN_OVERFLOW = 1u };
+
+// Device counters of one context (zero between frames).
+struct Counters {
+    uint32_t npairs;     // (triangle, tile) pairs emitted by k_setup
+    uint32_t flags;      // FrameFlags
+    uint32_t done;       // k_setup workgroups finished (last one scans)
+    uint32_t pad[13];
+};
 
 struct FrameParams {
     // target (tgt_size) and the two targets; null = Empty
@@ -58,6 +64,7 @@ struct FrameParams {
     uint32_t cull, msaa_level, band_par, G;
     uint32_t clear_color_en, clear_color, clear_depth_en;
     float clear_depth;
+    float ndc_x, ndc_y;   // 2/size_x and -2/size_y of to_ndc (triangles.rs:44-48)
     // mesh (device)
     const float *pos;
     const float *nrm;
@@ -66,31 +73,26 @@ struct FrameParams {
     const uint32_t *rgba8;
     const uint32_t *idx;
     const float *models;
-    uint32_t pos_comps, count, n_vertices, n_instances, n_tris, nvd;
+    uint32_t pos_comps, count, n_vertices, n_instances, n_tris;
     // textures (device)
     const uint32_t *tex;
     uint32_t tex_w, tex_h;
     const float *shadow;
     uint32_t sh_w, sh_h;
     // per-frame scratch (device)
-    float4 *clip;
-    float *vary;
-    TriRec *recs;
-    uint32_t *nrec;
-    uint32_t *tile_count;
-    uint32_t *tile_off;
-    uint32_t *tile_cursor;
+    TriRec *recs;          // indexed by primitive
+    Counters *ctr;
+    uint32_t *tile_count;  // per tile, zero between frames
+    uint32_t *tile_off;    // ntiles + 1
+    uint32_t *pair_prim, *pair_tile, *pair_rank;
     uint32_t *bins;
     uint32_t bin_cap;
-    uint32_t *flags;
     uint32_t ntx, nty;
     float u[kUniforms];
 };
 
 // Kernel launchers (kernels.hip).  All enqueue on `s`.
-hipError_t launch_vertex(const FrameParams &p, hipStream_t s);
 hipError_t launch_setup(const FrameParams &p, hipStream_t s);
-hipError_t launch_scan(const FrameParams &p, hipStream_t s);
 hipError_t launch_scatter(const FrameParams &p, hipStream_t s);
 hipError_t launch_raster(const FrameParams &p, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s);

@@ -1,111 +1,61 @@
 // kernels.hip -- gfx950 kernels of the triangle path.
 //
-// Frame = vertex -> setup -> scan -> scatter -> raster:
-//   k_vertex  one lane per (instance, vertex): the fixed vertex shader, clip
-//             positions (float4) and VertexData written once per vertex.
-//   k_setup   one lane per triangle: euc's triangle setup (triangles.rs:55-171)
-//             in IEEE f32, cull, compaction of survivors into 128-byte TriRec
-//             records, per-tile triangle counts.
-//   k_scan    per-tile counts -> bin offsets (single workgroup).
-//   k_scatter one lane per record (a whole wave for records spanning many
-//             tiles): bin lists of record ids per 8x8 tile.
-//   k_raster  one wave per 8x8 tile, one pixel per lane: coverage, z, z-clip
-//             and the depth rule resolved as an order-independent reduction
-//             (the winner of each pixel is a function of the set of its
-//             fragments with primitive ids as tie-breaks), then the winner is
-//             shaded once and colour + depth are written once.
+// Frame = setup -> scatter -> raster (three launches):
+//   k_setup   one lane per triangle: vertex positions (the fixed vertex shader),
+//             euc's triangle setup (triangles.rs:55-171) in IEEE f32, culling,
+//             a 128-byte record at the primitive's index, and one
+//             (primitive, tile, rank) pair per covered 32x8 tile, the rank
+//             taken from a per-tile atomic counter.  The last workgroup to
+//             finish turns the per-tile counts into bin offsets.
+//   k_scatter one lane per pair: bins[offset[tile] + rank] = primitive.
+//   k_raster  one 256-thread workgroup per 32x8 tile, one pixel per thread.
+//             Each pixel keeps a 64-bit key in LDS that encodes the winning
+//             fragment; the depth rule becomes an order-independent reduction
+//             (atomic min/max of the key) because every fixed shader's blend
+//             replaces the old pixel: the winner is a function of the set of
+//             fragments, with primitive order as the tie-break (see
+//             init_key).  Triangles that touch few pixels of the tile are
+//             rasterised one per lane, walking their rows exactly as
+//             triangles.rs:219-303 does; larger ones are tested pixel-parallel.
+//             The winner is then shaded once and colour and depth are written
+//             once, in full 128-byte rows.
 #include "internal.hpp"
 #include "shade.hpp"
 
 namespace euchip {
 
-EUCHIP_DEV uint32_t lane_id() {
-    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+EUCHIP_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+EUCHIP_DEV bool passes_z_clip(const FrameParams &p, float z) {
+    return !p.z_clip || (p.z_min <= z && z <= p.z_max);
 }
 
-// ---------------------------------------------------------------------------
-// vertex shading (Pipeline::vertex of the fixed shader set)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_vertex(FrameParams p) {
-    const uint32_t sid = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t total = p.n_instances * p.n_vertices;
-    if (sid >= total) return;
-    const uint32_t inst = sid / p.n_vertices;
-    const uint32_t vi = sid - inst * p.n_vertices;
-    const float *P = p.pos + (size_t)p.pos_comps * vi;
-    const float *M = p.models + 16u * inst;
-    float clip[4], wpos[4] = {0.f, 0.f, 0.f, 0.f}, wn[4] = {0.f, 0.f, 0.f, 0.f};
-    if (p.xform == 0) {
-        clip[0] = P[0], clip[1] = P[1], clip[2] = P[2];
-        clip[3] = p.pos_comps == 4 ? P[3] : 1.0f;
-        wpos[0] = clip[0], wpos[1] = clip[1], wpos[2] = clip[2];
-    } else {
-        float p4[4] = {P[0], P[1], P[2], 1.0f};
-        mat4_vec4(M, p4, wpos);
-        mat4_vec4(p.u + U_VP, wpos, clip);
+// Inclusive wave64 prefix sum.
+EUCHIP_DEV uint32_t wave_inclusive_sum(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += o;
     }
-    if (p.nrm) {
-        const float *N = p.nrm + 3u * vi;
-        float n4[4] = {N[0], N[1], N[2], 0.0f};
-        if (p.shader == PHONG_SHADOW) n4[0] = -N[0], n4[1] = -N[1], n4[2] = -N[2];
-        if (p.xform == 0) wn[0] = n4[0], wn[1] = n4[1], wn[2] = n4[2], wn[3] = n4[3];
-        else mat4_vec4(M, n4, wn);
-    }
-    p.clip[sid] = make_float4(clip[0], clip[1], clip[2], clip[3]);
-    float *vd = p.vary + (size_t)sid * p.nvd;
-    switch (p.shader) {
-    case FLAT: vd[0] = __uint_as_float(p.rgba8[vi]); break;
-    case COLOR:
-        for (int c = 0; c < 4; c++) vd[c] = p.col[4u * vi + c];
-        break;
-    case GOURAUD: {
-        float o[4];
-        lighting2(p.u, wpos, wn, o);
-        vd[0] = o[0], vd[1] = o[1], vd[2] = o[2], vd[3] = o[3];
-        break;
-    }
-    case PHONG:
-        vd[0] = wpos[0], vd[1] = wpos[1], vd[2] = wpos[2];
-        vd[3] = wn[0], vd[4] = wn[1], vd[5] = wn[2];
-        break;
-    case TEXTURED:
-        vd[0] = p.uv[2u * vi], vd[1] = p.uv[2u * vi + 1];
-        break;
-    case PHONG_SHADOW:
-    case TEX_SHADOW: {
-        float w1[4] = {wpos[0], wpos[1], wpos[2], 1.0f}, l4[4];
-        mat4_vec4(p.u + U_LVP, w1, l4);
-        vd[0] = wpos[0], vd[1] = wpos[1], vd[2] = wpos[2];
-        vd[3] = wn[0], vd[4] = wn[1], vd[5] = wn[2];
-        int o = 6;
-        if (p.shader == TEX_SHADOW) {
-            vd[6] = p.uv[2u * vi], vd[7] = p.uv[2u * vi + 1];
-            o = 8;
-        }
-        vd[o + 0] = l4[0] / l4[3];
-        vd[o + 1] = l4[1] / l4[3];
-        vd[o + 2] = l4[2] / l4[3];
-        break;
-    }
-    default: break;
-    }
+    return v;
 }
 
 // ---------------------------------------------------------------------------
 // Visit every tile of each active lane's tile rectangle.  Small rectangles are
-// walked by their own lane; rectangles with more than 8 tiles are spread over
-// the whole wave so one huge triangle does not serialise a lane.  f(tile, id)
-// receives the `id` of the lane that owns the rectangle.
+// walked by their own lane; rectangles of more than 8 tiles are spread over the
+// whole wave so one huge triangle does not serialise a lane.  f(tile, j, a, b)
+// gets the tile, its index j in the owner's rectangle and the owner's (a, b).
 // ---------------------------------------------------------------------------
 template <class F>
 EUCHIP_DEV void for_each_tile(bool act, uint32_t tx0, uint32_t ty0, uint32_t nx, uint32_t ny, uint32_t ntx,
-                              uint32_t id, F f) {
+                              uint32_t a, uint32_t b, F f) {
     const uint32_t n = act ? nx * ny : 0u;
     const bool big = n > 8u;
     if (!big) {
         for (uint32_t j = 0; j < n; j++) {
             uint32_t jy = j / nx, jx = j - jy * nx;
-            f((ty0 + jy) * ntx + tx0 + jx, id);
+            f((ty0 + jy) * ntx + tx0 + jx, j, a, b);
         }
     }
     uint64_t m = __ballot(big);
@@ -117,16 +67,63 @@ EUCHIP_DEV void for_each_tile(bool act, uint32_t tx0, uint32_t ty0, uint32_t nx,
         const uint32_t bty0 = __builtin_amdgcn_readlane(ty0, l);
         const uint32_t bnx = __builtin_amdgcn_readlane(nx, l);
         const uint32_t bn = __builtin_amdgcn_readlane(n, l);
-        const uint32_t bid = __builtin_amdgcn_readlane(id, l);
+        const uint32_t ba = __builtin_amdgcn_readlane(a, l);
+        const uint32_t bb = __builtin_amdgcn_readlane(b, l);
         for (uint32_t j = lane; j < bn; j += 64u) {
             uint32_t jy = j / bnx, jx = j - jy * bnx;
-            f((bty0 + jy) * ntx + btx0 + jx, bid);
+            f((bty0 + jy) * ntx + btx0 + jx, j, ba, bb);
         }
     }
 }
 
-EUCHIP_DEV bool passes_z_clip(const FrameParams &p, float z) {
-    return !p.z_clip || (p.z_min <= z && z <= p.z_max);
+// Bin offsets from per-tile counts, run by the last k_setup workgroup (256
+// threads).  Counts are read with device-scope atomic loads (they were
+// produced by device-scope atomics of every workgroup) and reset to zero for
+// the next frame.
+EUCHIP_DEV void scan_tile_counts(const FrameParams &p) {
+    __shared__ uint32_t buf[4096];
+    __shared__ uint32_t part[256];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n = p.ntx * p.nty;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += 4096u) {
+        const uint32_t m = n - base < 4096u ? n - base : 4096u;
+        for (uint32_t i = tid; i < m; i += 256u) {
+            buf[i] = __hip_atomic_load(p.tile_count + base + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            p.tile_count[base + i] = 0u;
+        }
+        __syncthreads();
+        const uint32_t per = (m + 255u) / 256u;
+        const uint32_t b0 = tid * per < m ? tid * per : m;
+        const uint32_t b1 = b0 + per < m ? b0 + per : m;
+        uint32_t s = 0;
+        for (uint32_t i = b0; i < b1; i++) s += buf[i];
+        part[tid] = s;
+        __syncthreads();
+        for (uint32_t d = 1; d < 256u; d <<= 1) {
+            uint32_t v = tid >= d ? part[tid - d] : 0u;
+            __syncthreads();
+            part[tid] += v;
+            __syncthreads();
+        }
+        uint32_t run = carry + part[tid] - s;
+        for (uint32_t i = b0; i < b1; i++) {
+            uint32_t c = buf[i];
+            buf[i] = run;
+            run += c;
+        }
+        const uint32_t total = part[255];
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += 256u) p.tile_off[base + i] = buf[i];
+        carry += total;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        p.tile_off[n] = carry;
+        const uint32_t np = __hip_atomic_load(&p.ctr->npairs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (np > p.bin_cap || carry != np) atomicOr(&p.ctr->flags, FLAG_BIN_OVERFLOW);
+        p.ctr->done = 0u;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -138,7 +135,7 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
     TriRec R;
     if (t < p.n_tris) {
         float hom[3][4], euc[3][3];
-        uint32_t sid[3];
+        uint32_t vis[3], ins[3];
         const float flip1 = p.y_up ? -1.0f : 1.0f;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -146,12 +143,14 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
             const uint32_t inst = e / p.count;
             const uint32_t k = e - inst * p.count;
             const uint32_t vi = p.idx ? p.idx[k] : k;
-            sid[i] = inst * p.n_vertices + vi;
-            const float4 c = p.clip[sid[i]];
-            hom[i][0] = c.x * 1.0f;
-            hom[i][1] = c.y * flip1;
-            hom[i][2] = c.z;
-            hom[i][3] = c.w;
+            vis[i] = vi;
+            ins[i] = inst;
+            float c[4];
+            vertex_clip(p, inst, vi, c);
+            hom[i][0] = c[0] * 1.0f;
+            hom[i][1] = c[1] * flip1;
+            hom[i][2] = c[2];
+            hom[i][3] = c[3];
         }
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -180,12 +179,15 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
                     euc[0][c] = euc[2][c];
                     euc[2][c] = tmp;
                 }
-                uint32_t ts = sid[0];
-                sid[0] = sid[2];
-                sid[2] = ts;
+                uint32_t ts = vis[0];
+                vis[0] = vis[2];
+                vis[2] = ts;
+                ts = ins[0];
+                ins[0] = ins[2];
+                ins[2] = ts;
             }
             const float sx = (float)p.W, sy = (float)p.H;
-            const float T[3][3] = {{2.0f / sx, 0.0f, -1.0f}, {0.0f, -2.0f / sy, 1.0f}, {0.0f, 0.0f, 1.0f}};
+            const float T[3][3] = {{p.ndc_x, 0.0f, -1.0f}, {0.0f, p.ndc_y, 1.0f}, {0.0f, 0.0f, 1.0f}};
             // coords_to_weights (triangles.rs:86-102)
             const float c3[3] = {hom[2][0], hom[2][1], hom[2][3]};
             const float ca[3] = {hom[0][0] - c3[0], hom[0][1] - c3[1], hom[0][3] - c3[2]};
@@ -208,7 +210,8 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
 #pragma unroll
                 for (int j = 0; j < 3; j++) M[i][j] = rows[i][0] * T[0][j] + rows[i][1] * T[1][j] + rows[i][2] * T[2][j];
             }
-            // screen coordinates and bounds (triangles.rs:110-139), band clamp = whole target
+            // screen coordinates and bounds (triangles.rs:110-139); the band clamp of
+            // y is applied per row by k_raster, here the clamp is the whole target
             float vs[3][2];
 #pragma unroll
             for (int i = 0; i < 3; i++) {
@@ -229,7 +232,8 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
                 R.dx[i] = (wx - o) * (1.0f / 1000.0f);
                 R.dy[i] = (wy - o) * (1.0f / 1000.0f);
                 R.hz[i] = hom[i][2];
-                R.sid[i] = sid[i];
+                R.vi[i] = vis[i];
+                R.inst[i] = ins[i];
             }
             // verts_by_y (triangles.rs:148-171), ties resolved as the reference does
             const float min_y = rmin(rmin(vs[0][1], vs[1][1]), vs[2][1]);
@@ -249,131 +253,195 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
             R.vby[4] = vs[ic][0], R.vby[5] = vs[ic][1];
             const bool allin = passes_z_clip(p, euc[0][2]) && passes_z_clip(p, euc[1][2]) && passes_z_clip(p, euc[2][2]);
             R.flags = allin ? 1u : 0u;
-            R.prim = t;
-            R.pad[0] = R.pad[1] = R.pad[2] = R.pad[3] = R.pad[4] = 0u;
+            R.pad[0] = R.pad[1] = R.pad[2] = 0u;
             keep = R.xmin < R.xmax && R.ymin < R.ymax;
         }
     }
-    // wave-aggregated compaction of the surviving records
-    const uint64_t mask = __ballot(keep);
-    const uint32_t lane = lane_id();
-    uint32_t base = 0;
-    if (mask) {
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        if ((int)lane == leader) base = atomicAdd(p.nrec, (uint32_t)__popcll(mask));
-        base = __builtin_amdgcn_readlane(base, leader);
-    }
-    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
     uint32_t tx0 = 0, ty0 = 0, nx = 0, ny = 0;
     if (keep) {
-        const uint32_t r = base + rank;
-        uint4 *dst = reinterpret_cast<uint4 *>(p.recs + r);
+        uint4 *dst = reinterpret_cast<uint4 *>(p.recs + t);
         const uint4 *src = reinterpret_cast<const uint4 *>(&R);
 #pragma unroll
         for (int q = 0; q < 8; q++) dst[q] = src[q];
-        tx0 = R.xmin / kTileW;
-        ty0 = R.ymin / kTileH;
-        nx = (R.xmax - 1u) / kTileW - tx0 + 1u;
-        ny = (R.ymax - 1u) / kTileH - ty0 + 1u;
+        tx0 = R.xmin >> kTileShiftX;
+        ty0 = R.ymin >> kTileShiftY;
+        nx = ((R.xmax - 1u) >> kTileShiftX) - tx0 + 1u;
+        ny = ((R.ymax - 1u) >> kTileShiftY) - ty0 + 1u;
     }
+    // Pair slots: one wave-aggregated allocation, an exclusive prefix per lane.
+    const uint32_t n = keep ? nx * ny : 0u;
+    const uint32_t incl = wave_inclusive_sum(n);
+    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+    uint32_t base = 0;
+    if (total) {
+        if (lane_id() == 0u) base = atomicAdd(&p.ctr->npairs, total);
+        base = __builtin_amdgcn_readfirstlane(base);
+    }
+    const uint32_t pbase = base + incl - n;
     uint32_t *cnt = p.tile_count;
-    for_each_tile(keep, tx0, ty0, nx, ny, p.ntx, 0u, [&](uint32_t tile, uint32_t) { atomicAdd(cnt + tile, 1u); });
-}
-
-// ---------------------------------------------------------------------------
-// per-tile counts -> bin offsets; counts are reset for the next frame
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_scan(FrameParams p) {
-    __shared__ uint32_t part[1024];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t n = p.ntx * p.nty;
-    const uint32_t per = (n + 1023u) / 1024u;
-    const uint32_t beg = tid * per < n ? tid * per : n;
-    const uint32_t end = beg + per < n ? beg + per : n;
-    uint32_t sum = 0;
-    for (uint32_t i = beg; i < end; i++) sum += p.tile_count[i];
-    part[tid] = sum;
+    const uint32_t cap = p.bin_cap;
+    for_each_tile(keep, tx0, ty0, nx, ny, p.ntx, t, pbase, [&](uint32_t tile, uint32_t j, uint32_t prim, uint32_t pb) {
+        const uint32_t rank = atomicAdd(cnt + tile, 1u);
+        const uint32_t i = pb + j;
+        if (i < cap) {
+            p.pair_prim[i] = prim;
+            p.pair_tile[i] = tile;
+            p.pair_rank[i] = rank;
+        }
+    });
+    // The last workgroup to finish builds the bin offsets.
+    __shared__ uint32_t s_last;
+    __threadfence();
     __syncthreads();
-    for (uint32_t off = 1; off < 1024u; off <<= 1) {
-        uint32_t v = tid >= off ? part[tid - off] : 0u;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[tid] - sum;
-    for (uint32_t i = beg; i < end; i++) {
-        const uint32_t c = p.tile_count[i];
-        p.tile_off[i] = run;
-        p.tile_cursor[i] = run;
-        p.tile_count[i] = 0u;
-        run += c;
-    }
-    if (tid == 1023u) {
-        p.tile_off[n] = part[1023];
-        if (part[1023] > p.bin_cap) atomicOr(p.flags, FLAG_BIN_OVERFLOW);
+    if (threadIdx.x == 0) s_last = atomicAdd(&p.ctr->done, 1u) == gridDim.x - 1u ? 1u : 0u;
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        scan_tile_counts(p);
     }
 }
 
 // ---------------------------------------------------------------------------
-// bin lists: record ids per tile (order inside a bin is irrelevant, see k_raster)
+// bins[offset[tile] + rank] = primitive (order inside a bin is irrelevant)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_scatter(FrameParams p) {
-    if (p.flags[0] & FLAG_BIN_OVERFLOW) return;
-    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t nrec = p.nrec[0];
-    const bool act = r < nrec;
-    uint32_t tx0 = 0, ty0 = 0, nx = 0, ny = 0;
-    if (act) {
-        const TriRec &R = p.recs[r];
-        tx0 = R.xmin / kTileW;
-        ty0 = R.ymin / kTileH;
-        nx = (R.xmax - 1u) / kTileW - tx0 + 1u;
-        ny = (R.ymax - 1u) / kTileH - ty0 + 1u;
-    }
-    uint32_t *cur = p.tile_cursor;
-    uint32_t *bins = p.bins;
-    const uint32_t cap = p.bin_cap;
-    for_each_tile(act, tx0, ty0, nx, ny, p.ntx, r, [&](uint32_t tile, uint32_t owner) {
-        const uint32_t pos = atomicAdd(cur + tile, 1u);
-        if (pos < cap) bins[pos] = owner;
-    });
+    if (p.ctr->flags & FLAG_BIN_OVERFLOW) return;
+    const uint32_t n = p.ctr->npairs;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
+        p.bins[p.tile_off[p.pair_tile[i]] + p.pair_rank[i]] = p.pair_prim[i];
 }
 
 // ---------------------------------------------------------------------------
-// deferred shading of one pixel's winning fragment (pipeline.rs:529-578)
+// raster
 // ---------------------------------------------------------------------------
-EUCHIP_DEV void v_data(const FrameParams &p, const TriRec &R, const float *V0, const float *V1, const float *V2,
-                       float X, float Y, float *vd) {
-    // get_v_data (triangles.rs:274-294) and weighted_sum3 (math.rs:37-40)
+
+// Total order of f32 for the winner keys; -0 and +0 map to one value.
+EUCHIP_DEV uint32_t zord(float z) {
+    const uint32_t u = __float_as_uint(z);
+    if ((u & 0x7fffffffu) == 0u) return 0x80000000u;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+EUCHIP_DEV float zunord(uint32_t o, uint32_t negzero) {
+    if (o == 0x80000000u) return negzero ? -0.0f : 0.0f;
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+// Winner keys.  mode 1 (test Less, write): min of (zord(z), prim+1, sign of zero)
+// = the first fragment of the smallest z below D0.  mode 2 (Greater, write):
+// the same with ~zord.  mode 0 (every other DepthMode): max of (prim+1, z bits)
+// over fragments passing the test against D0 = the last passing fragment.
+// In the reference's sequential order (pipeline.rs:519-538) these are exactly
+// the fragments whose emit_fragment determines the final pixel and depth.
+EUCHIP_DEV uint64_t init_key(uint32_t mode, float D0) {
+    if (mode == 0u) return 0ull;
+    if (D0 != D0) return 0ull;  // nothing compares with NaN
+    const uint32_t o = mode == 1u ? zord(D0) : ~zord(D0);
+    return (uint64_t)o << 32;
+}
+
+EUCHIP_DEV void update_key(uint64_t *key, uint32_t mode, uint32_t test, float z, float D0, uint32_t prim) {
+    if (mode == 0u) {
+        bool pass;
+        switch (test) {
+        case DT_LESS: pass = z < D0; break;
+        case DT_EQUAL: pass = z == D0; break;
+        case DT_GREATER: pass = z > D0; break;
+        default: pass = true; break;
+        }
+        if (pass) atomicMax((unsigned long long *)key, ((uint64_t)(prim + 1u) << 32) | __float_as_uint(z));
+    } else {
+        if (z != z) return;
+        const uint32_t o = mode == 1u ? zord(z) : ~zord(z);
+        const uint32_t nz = __float_as_uint(z) == 0x80000000u ? 1u : 0u;
+        atomicMin((unsigned long long *)key, ((uint64_t)o << 32) | ((prim + 1u) << 1) | nz);
+    }
+}
+
+// Row span [x0, x1) of row y for a triangle (triangles.rs:219-254); the
+// "stupid version" switch uses the band-clamped bounding box.
+EUCHIP_DEV void row_span(const TriRec &R, uint32_t y, uint32_t by0, uint32_t by1, uint32_t &x0, uint32_t &x1) {
+    const uint32_t bw = R.xmax - R.xmin;
+    const uint32_t yl = R.ymin < by0 ? by0 : (R.ymin > by1 ? by1 : R.ymin);
+    const uint32_t yh = R.ymax < by0 ? by0 : (R.ymax > by1 ? by1 : R.ymax);
+    const uint32_t bh = yh - yl;
+    if (bw < 128u && bh < 128u && bw * bh < 128u) {
+        x0 = R.xmin;
+        x1 = R.xmax;
+        return;
+    }
+    const float Y = (float)y;
+    const float ax = R.vby[0], ay = R.vby[1], bx = R.vby[2], byy = R.vby[3], cx = R.vby[4], cy = R.vby[5];
+    const float ac = ax + ((Y - ay) / (cy - ay)) * (cx - ax);
+    float lo, hi;
+    if (Y < byy) {
+        const float ab = ax + ((Y - ay) / (byy - ay)) * (bx - ax);
+        lo = rmin(ab, ac);
+        hi = rmax(ab, ac);
+    } else {
+        const float bc = bx + ((Y - byy) / (cy - byy)) * (cx - bx);
+        lo = rmin(bc, ac);
+        hi = rmax(bc, ac);
+    }
+    const float fl = floorf(lo), ce = ceilf(hi);
+    const float fxmin = (float)R.xmin, fxmax = (float)R.xmax;
+    x0 = (fl >= fxmin && fl < fxmax) ? (uint32_t)fl : R.xmin;
+    x1 = (ce >= fxmin && ce < fxmax) ? (uint32_t)ce : R.xmax;
+}
+
+EUCHIP_DEV void band_of(const FrameParams &p, uint32_t y, uint32_t &by0, uint32_t &by1) {
+    if (p.band_par) {
+        by0 = (y / p.G) * p.G;
+        by1 = by0 + p.G < p.H ? by0 + p.G : p.H;
+    } else {
+        by0 = 0;
+        by1 = p.H;
+    }
+}
+
+EUCHIP_DEV void load_rec(const TriRec *recs, uint32_t prim, TriRec &R) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(recs + prim);
+    uint4 *dst = reinterpret_cast<uint4 *>(&R);
+#pragma unroll
+    for (int q = 0; q < 8; q++) dst[q] = src[q];
+}
+
+template <uint32_t SH>
+struct VD {
+    static constexpr uint32_t N = vd_count(SH) ? vd_count(SH) : 1;
+};
+
+// get_v_data (triangles.rs:274-294) and weighted_sum3 (math.rs:37-40)
+template <uint32_t SH>
+EUCHIP_DEV void v_data(const TriRec &R, const float (*V)[VD<SH>::N], float X, float Y, float *vd) {
     float w[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) w[i] = (R.o[i] + R.dy[i] * Y) + R.dx[i] * X;
     const float u2 = (w[2] - w[0]) - w[1];
     const float r = 1.0f / w[2];
     const float n0 = w[0] * r, n1 = w[1] * r, n2 = u2 * r;
-    if (p.shader == FLAT) {
-        vd[0] = V0[0];
-        return;
+    if constexpr (SH == FLAT) {
+        vd[0] = V[0][0];
+    } else {
+#pragma unroll
+        for (uint32_t c = 0; c < vd_count(SH); c++) vd[c] = (V[0][c] * n0 + V[1][c] * n1) + V[2][c] * n2;
     }
-    for (uint32_t c = 0; c < p.nvd; c++) vd[c] = (V0[c] * n0 + V1[c] * n1) + V2[c] * n2;
 }
 
+// Deferred shading of one pixel's winning fragment (pipeline.rs:539-576).
+template <uint32_t SH>
 EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const TriRec &R, uint32_t px, uint32_t py, uint32_t by0,
                                 uint32_t old) {
-    float V[3][kMaxVD];
-    const uint32_t nvd = p.nvd;
+    constexpr uint32_t NV = VD<SH>::N;
+    float V[3][NV];
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const float *src = p.vary + (size_t)R.sid[i] * nvd;
-        for (uint32_t c = 0; c < nvd; c++) V[i][c] = src[c];
-    }
-    float frag[4], vd[kMaxVD];
+    for (int i = 0; i < 3; i++) vertex_data<SH>(p, R.inst[i], R.vi[i], V[i]);
+    float frag[4], vd[NV];
     const uint32_t L = p.msaa_level;
     if (L == 0u) {
-        v_data(p, R, V[0], V[1], V[2], (float)px, (float)py, vd);
-        fragment_shader(p, vd, frag);
+        v_data<SH>(R, V, (float)px, (float)py, vd);
+        fragment_shader<SH>(p, vd, frag);
     } else {
-        // coarse shading on the band-relative 2^L grid, bilinear reconstruction
+        // coarse shading on the band-relative 2^L grid with bilinear reconstruction
         // (pipeline.rs:541-570); tgt_min = (0, by0)
         const float div = 1.0f / (float)(1u << L);
         const uint32_t rx = px, ry = py - by0;
@@ -384,10 +452,10 @@ EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const TriRec &R, uint32_t 
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const uint32_t ix = cx + (uint32_t)(q & 1), iy = cy + (uint32_t)(q >> 1);
-            v_data(p, R, V[0], V[1], V[2], (float)(ix << L), (float)(by0 + (iy << L)), vd);
-            fragment_shader(p, vd, t[q]);
+            v_data<SH>(R, V, (float)(ix << L), (float)(by0 + (iy << L)), vd);
+            fragment_shader<SH>(p, vd, t[q]);
         }
-        if (p.shader == FLAT) {
+        if constexpr (SH == FLAT) {
             frag[0] = t[0][0];
         } else {
 #pragma unroll
@@ -398,123 +466,165 @@ EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const TriRec &R, uint32_t 
             }
         }
     }
-    return blend_shader(p, old, frag);
+    return blend_shader<SH>(old, frag);
 }
 
-// ---------------------------------------------------------------------------
-// raster: one 8x8 tile per wave64 workgroup
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_raster(FrameParams p) {
-    if (p.flags[0] & FLAG_BIN_OVERFLOW) return;
-    const uint32_t tile = blockIdx.x;
-    const uint32_t lane = threadIdx.x;
-    if (tile == 0u && lane == 0u) p.nrec[0] = 0u;  // record counter of the next frame
-    const uint32_t ty = tile / p.ntx, tx = tile - ty * p.ntx;
-    const uint32_t px = tx * kTileW + (lane & 7u);
-    const uint32_t py = ty * kTileH + (lane >> 3);
-    const bool inside = px < p.W && py < p.H;
-    const size_t pidx = (size_t)py * p.W + px;
-    uint32_t by0 = 0, by1 = p.H;
-    if (p.band_par) {
-        by0 = (py / p.G) * p.G;
-        by1 = by0 + p.G < p.H ? by0 + p.G : p.H;
-    }
-    const bool uses_depth = p.depth_test != DT_NONE || p.depth_write;
-    float D0 = 0.0f;
-    if (inside && uses_depth) D0 = p.clear_depth_en ? p.clear_depth : p.depth[pidx];
-    // 0: winner = last fragment passing the test against D0
-    // 1: LESS + write: winner = first fragment of minimum z below D0
-    // 2: GREATER + write: winner = first fragment of maximum z above D0
-    const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
-                        : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
-    float best_z = D0;
-    uint32_t best_prim = 0u, best_rec = 0u;
-    bool has = false;
-    const uint32_t beg = p.tile_off[tile], end = p.tile_off[tile + 1u];
-    for (uint32_t k = beg; k < end; k++) {
-        const uint32_t r = p.bins[k];
-        const TriRec &R = p.recs[r];
-        if (!inside || py < R.ymin || py >= R.ymax) continue;
-        const uint32_t bw = R.xmax - R.xmin;
-        const uint32_t yl = R.ymin < by0 ? by0 : (R.ymin > by1 ? by1 : R.ymin);
-        const uint32_t yh = R.ymax < by0 ? by0 : (R.ymax > by1 ? by1 : R.ymax);
-        const uint32_t bh = yh - yl;
-        uint32_t x0, x1;
-        if (bw < 128u && bh < 128u && bw * bh < 128u) {
-            x0 = R.xmin;
-            x1 = R.xmax;
-        } else {
-            // scanline row range (triangles.rs:228-253)
-            const float Y = (float)py;
-            const float ax = R.vby[0], ay = R.vby[1], bx = R.vby[2], byy = R.vby[3], cx = R.vby[4], cy = R.vby[5];
-            const float ac = ax + ((Y - ay) / (cy - ay)) * (cx - ax);
-            float lo, hi;
-            if (Y < byy) {
-                const float ab = ax + ((Y - ay) / (byy - ay)) * (bx - ax);
-                lo = rmin(ab, ac);
-                hi = rmax(ab, ac);
-            } else {
-                const float bc = bx + ((Y - byy) / (cy - byy)) * (cx - bx);
-                lo = rmin(bc, ac);
-                hi = rmax(bc, ac);
-            }
-            const float fl = floorf(lo), ce = ceilf(hi);
-            const float fxmin = (float)R.xmin, fxmax = (float)R.xmax;
-            x0 = (fl >= fxmin && fl < fxmax) ? (uint32_t)fl : R.xmin;
-            x1 = (ce >= fxmin && ce < fxmax) ? (uint32_t)ce : R.xmax;
-        }
-        if (px < x0 || px >= x1) continue;
-        // w_hom at the row start, then the reference's sequential w += dx walk
-        const float Y = (float)py, X0 = (float)x0;
+// Fragment test of one pixel (triangles.rs:262-301): weights, coverage, z, z-clip.
+EUCHIP_DEV bool frag_z(const FrameParams &p, const TriRec &R, float w0, float w1, float w2, float &z) {
+    const float u2 = (w2 - w0) - w1;
+    if (!(w0 >= 0.0f && w1 >= 0.0f && u2 >= 0.0f)) return false;
+    z = (R.hz[0] * w0 + R.hz[1] * w1) + R.hz[2] * u2;
+    return (R.flags & 1u) || passes_z_clip(p, z);
+}
+
+// A lane rasterises one triangle over the rows it has in this tile, walking
+// each row from the row start exactly as the reference does.
+EUCHIP_DEV void raster_lane(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t X0t, uint32_t Y0t,
+                            uint32_t mode, uint64_t *key, const float *d0s) {
+    const uint32_t ry0 = R.ymin > Y0t ? R.ymin : Y0t;
+    const uint32_t ry1 = R.ymax < Y0t + kTileH ? R.ymax : Y0t + kTileH;
+    for (uint32_t y = ry0; y < ry1; y++) {
+        uint32_t by0, by1, x0, x1;
+        band_of(p, y, by0, by1);
+        row_span(R, y, by0, by1, x0, x1);
+        const uint32_t xs = x0 > X0t ? x0 : X0t;
+        const uint32_t xe = x1 < X0t + kTileW ? x1 : X0t + kTileW;
+        if (xs >= xe) continue;
+        const float Y = (float)y, X0 = (float)x0;
         float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
         float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
         float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
-        for (uint32_t x = x0; x < px; x++) {
+        for (uint32_t x = x0; x < xs; x++) {
             w0 = w0 + R.dx[0];
             w1 = w1 + R.dx[1];
             w2 = w2 + R.dx[2];
         }
-        const float u2 = (w2 - w0) - w1;
-        if (!(w0 >= 0.0f && w1 >= 0.0f && u2 >= 0.0f)) continue;
-        const float z = (R.hz[0] * w0 + R.hz[1] * w1) + R.hz[2] * u2;
-        if (!((R.flags & 1u) || passes_z_clip(p, z))) continue;
-        bool acc;
-        if (mode == 1u) {
-            acc = z < best_z || (z == best_z && has && R.prim < best_prim);
-        } else if (mode == 2u) {
-            acc = z > best_z || (z == best_z && has && R.prim < best_prim);
-        } else {
-            bool pass;
-            switch (p.depth_test) {
-            case DT_LESS: pass = z < D0; break;
-            case DT_EQUAL: pass = z == D0; break;
-            case DT_GREATER: pass = z > D0; break;
-            default: pass = true; break;
+        const uint32_t row = (y - Y0t) * kTileW;
+        for (uint32_t x = xs; x < xe; x++) {
+            float z;
+            if (frag_z(p, R, w0, w1, w2, z)) {
+                const uint32_t pi = row + (x - X0t);
+                update_key(key + pi, mode, p.depth_test, z, d0s[pi], prim);
             }
-            acc = pass && (!has || R.prim > best_prim);
-        }
-        if (acc) {
-            has = true;
-            best_z = z;
-            best_prim = R.prim;
-            best_rec = r;
+            w0 = w0 + R.dx[0];
+            w1 = w1 + R.dx[1];
+            w2 = w2 + R.dx[2];
         }
     }
-    if (!inside) return;
-    if (p.pixel) {
-        if (p.pixel_write && has) {
-            if (p.shader != DEPTH_ONLY) {
-                const uint32_t old = p.clear_color_en ? p.clear_color : 0u;
-                p.pixel[pidx] = shade_pixel(p, p.recs[best_rec], px, py, by0, old);
-            } else if (p.clear_color_en) {
-                p.pixel[pidx] = p.clear_color;
+}
+
+// Workgroup-to-tile map: the hardware deals workgroups round-robin to the 8
+// XCDs; give each XCD a contiguous run of tiles so triangles shared by
+// neighbouring tiles stay in that XCD's L2.
+EUCHIP_DEV uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    const uint32_t q = n >> 3, r = n & 7u;
+    const uint32_t x = b & 7u, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// Lanes whose triangle would cost more than this many row steps in the tile
+// hand it to the pixel-parallel pass.
+constexpr uint32_t kLaneWork = 48;
+
+template <uint32_t SH>
+__global__ __launch_bounds__(256) void k_raster(FrameParams p) {
+    __shared__ uint64_t key[kTileW * kTileH];
+    __shared__ float d0s[kTileW * kTileH];
+    __shared__ uint32_t big[256];
+    __shared__ uint32_t nbig;
+    if (p.ctr->flags & FLAG_BIN_OVERFLOW) return;
+    const uint32_t ntiles = p.ntx * p.nty;
+    const uint32_t tile = xcd_tile(blockIdx.x, ntiles);
+    const uint32_t tid = threadIdx.x;
+    if (tile == 0u && tid == 0u) p.ctr->npairs = 0u;  // pair counter of the next frame
+    const uint32_t ty = tile / p.ntx, tx = tile - ty * p.ntx;
+    const uint32_t X0t = tx * kTileW, Y0t = ty * kTileH;
+    const uint32_t px = X0t + (tid & (kTileW - 1)), py = Y0t + (tid >> kTileShiftX);
+    const bool inside = px < p.W && py < p.H;
+    const size_t pidx = (size_t)py * p.W + px;
+    const bool uses_depth = p.depth_test != DT_NONE || p.depth_write;
+    float D0 = 0.0f;
+    if (inside && uses_depth) D0 = p.clear_depth_en ? p.clear_depth : p.depth[pidx];
+    const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
+                        : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
+    key[tid] = init_key(mode, D0);
+    d0s[tid] = D0;
+    if (tid == 0u) nbig = 0u;
+    __syncthreads();
+
+    const uint32_t beg = p.tile_off[tile], end = p.tile_off[tile + 1u];
+    for (uint32_t b = beg; b < end; b += 256u) {
+        const uint32_t i = b + tid;
+        if (i < end) {
+            const uint32_t prim = p.bins[i];
+            TriRec R;
+            load_rec(p.recs, prim, R);
+            const uint32_t ry0 = R.ymin > Y0t ? R.ymin : Y0t;
+            const uint32_t ry1 = R.ymax < Y0t + kTileH ? R.ymax : Y0t + kTileH;
+            const uint32_t xe = R.xmax < X0t + kTileW ? R.xmax : X0t + kTileW;
+            const uint32_t cost = (ry1 > ry0 ? ry1 - ry0 : 0u) * (xe > R.xmin ? xe - R.xmin : 0u);
+            if (cost <= kLaneWork) {
+                raster_lane(p, R, prim, X0t, Y0t, mode, key, d0s);
+            } else {
+                big[atomicAdd(&nbig, 1u)] = prim;
             }
+        }
+        __syncthreads();
+        const uint32_t nb = nbig;
+        for (uint32_t k = 0; k < nb; k++) {
+            const uint32_t prim = __builtin_amdgcn_readfirstlane(big[k]);
+            TriRec R;
+            load_rec(p.recs, prim, R);
+            if (!inside || py < R.ymin || py >= R.ymax) continue;
+            uint32_t by0, by1, x0, x1;
+            band_of(p, py, by0, by1);
+            row_span(R, py, by0, by1, x0, x1);
+            if (px < x0 || px >= x1) continue;
+            const float Y = (float)py, X0 = (float)x0;
+            float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
+            float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
+            float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
+            for (uint32_t x = x0; x < px; x++) {
+                w0 = w0 + R.dx[0];
+                w1 = w1 + R.dx[1];
+                w2 = w2 + R.dx[2];
+            }
+            float z;
+            if (frag_z(p, R, w0, w1, w2, z)) update_key(key + tid, mode, p.depth_test, z, D0, prim);
+        }
+        __syncthreads();
+        if (tid == 0u) nbig = 0u;
+        __syncthreads();
+    }
+
+    if (!inside) return;
+    const uint64_t kk = key[tid];
+    bool has;
+    uint32_t prim;
+    float z;
+    if (mode == 0u) {
+        has = kk != 0ull;
+        prim = (uint32_t)(kk >> 32) - 1u;
+        z = __uint_as_float((uint32_t)kk);
+    } else {
+        has = ((uint32_t)kk) != 0u;
+        prim = ((uint32_t)kk >> 1) - 1u;
+        const uint32_t o = (uint32_t)(kk >> 32);
+        z = zunord(mode == 1u ? o : ~o, (uint32_t)kk & 1u);
+    }
+    if (p.pixel) {
+        if (p.pixel_write && has && SH != DEPTH_ONLY) {
+            uint32_t by0, by1;
+            band_of(p, py, by0, by1);
+            TriRec R;
+            load_rec(p.recs, prim, R);
+            p.pixel[pidx] = shade_pixel<SH>(p, R, px, py, by0, p.clear_color_en ? p.clear_color : 0u);
         } else if (p.clear_color_en) {
             p.pixel[pidx] = p.clear_color;
         }
     }
     if (p.depth) {
-        if (has && p.depth_write) p.depth[pidx] = best_z;
+        if (has && p.depth_write) p.depth[pidx] = z;
         else if (p.clear_depth_en) p.depth[pidx] = p.clear_depth;
     }
 }
@@ -529,30 +639,37 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *dst, uint32_t v, siz
 // ---------------------------------------------------------------------------
 static inline uint32_t blocks(size_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
-hipError_t launch_vertex(const FrameParams &p, hipStream_t s) {
-    const size_t n = (size_t)p.n_instances * p.n_vertices;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_vertex, dim3(blocks(n, 256)), dim3(256), 0, s, p);
-    return hipGetLastError();
-}
 hipError_t launch_setup(const FrameParams &p, hipStream_t s) {
     if (p.n_tris == 0) return hipSuccess;
     hipLaunchKernelGGL(k_setup, dim3(blocks(p.n_tris, 256)), dim3(256), 0, s, p);
     return hipGetLastError();
 }
-hipError_t launch_scan(const FrameParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, p);
-    return hipGetLastError();
-}
+
 hipError_t launch_scatter(const FrameParams &p, hipStream_t s) {
-    if (p.n_tris == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter, dim3(blocks(p.n_tris, 256)), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_scatter, dim3(1024), dim3(256), 0, s, p);
     return hipGetLastError();
 }
+
+template <uint32_t SH>
+static void launch_raster_t(const FrameParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_raster<SH>, dim3(p.ntx * p.nty), dim3(256), 0, s, p);
+}
+
 hipError_t launch_raster(const FrameParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_raster, dim3(p.ntx * p.nty), dim3(64), 0, s, p);
+    switch (p.shader) {
+    case FLAT: launch_raster_t<FLAT>(p, s); break;
+    case COLOR: launch_raster_t<COLOR>(p, s); break;
+    case GOURAUD: launch_raster_t<GOURAUD>(p, s); break;
+    case PHONG: launch_raster_t<PHONG>(p, s); break;
+    case DEPTH_ONLY: launch_raster_t<DEPTH_ONLY>(p, s); break;
+    case TEXTURED: launch_raster_t<TEXTURED>(p, s); break;
+    case PHONG_SHADOW: launch_raster_t<PHONG_SHADOW>(p, s); break;
+    case TEX_SHADOW: launch_raster_t<TEX_SHADOW>(p, s); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
+
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks(n, 256)), dim3(256), 0, s, dst, value, n);

@@ -49,13 +49,10 @@ struct euchip_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, Stat> stats;
     // per-frame scratch, grown on demand
-    DevArray<float4> clip;
-    DevArray<float> vary;
     DevArray<TriRec> recs;
-    DevArray<uint32_t> tile_count, tile_off, tile_cursor, bins;
-    uint32_t *counters = nullptr;  // [0] = record counter, [1] = frame flags
-    uint32_t *host_flags = nullptr;
-    bool deferred_overflow = false;
+    DevArray<uint32_t> tile_count, tile_off, pair_prim, pair_tile, pair_rank, bins;
+    Counters *counters = nullptr;
+    Counters *host_counters = nullptr;
 };
 
 struct euchip_buffer {
@@ -274,47 +271,50 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     p.pos = mesh->pos, p.nrm = mesh->nrm, p.col = mesh->col, p.uv = mesh->uv, p.rgba8 = mesh->rgba8;
     p.idx = mesh->idx, p.models = mesh->models;
     p.pos_comps = mesh->pos_comps, p.count = mesh->count, p.n_vertices = mesh->n_vertices;
-    p.n_instances = mesh->n_instances, p.n_tris = (uint32_t)n_tris, p.nvd = vd_count(sh);
+    p.n_instances = mesh->n_instances, p.n_tris = (uint32_t)n_tris;
+    p.ndc_x = 2.0f / (float)W;
+    p.ndc_y = -2.0f / (float)H;
     if (d->texture) p.tex = (const uint32_t *)d->texture->ptr, p.tex_w = d->texture->w, p.tex_h = d->texture->h;
     if (d->shadow) p.shadow = (const float *)d->shadow->ptr, p.sh_w = d->shadow->w, p.sh_h = d->shadow->h;
     memcpy(p.u, uniforms, sizeof(p.u));
     p.ntx = (W + kTileW - 1) / kTileW;
     p.nty = (H + kTileH - 1) / kTileH;
     const size_t ntiles = (size_t)p.ntx * p.nty;
-    const size_t nsv = (size_t)mesh->n_instances * mesh->n_vertices;
 
     euchip_status s;
-    if ((s = ensure(ctx, ctx->clip, nsv, "clip positions"))) return s;
-    if ((s = ensure(ctx, ctx->vary, std::max<size_t>(nsv * std::max<uint32_t>(p.nvd, 1u), 1), "varyings"))) return s;
     if ((s = ensure(ctx, ctx->recs, (size_t)n_tris, "triangle records"))) return s;
-    bool fresh_tiles = ctx->tile_count.cap < ntiles;
+    const bool fresh_tiles = ctx->tile_count.cap < ntiles;
     if ((s = ensure(ctx, ctx->tile_count, ntiles, "tile counts"))) return s;
     if (fresh_tiles) HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4, ctx->stream));
     if ((s = ensure(ctx, ctx->tile_off, ntiles + 1, "tile offsets"))) return s;
-    if ((s = ensure(ctx, ctx->tile_cursor, ntiles, "tile cursors"))) return s;
-    size_t want_bins = std::max<size_t>((size_t)4 << 20, (size_t)n_tris * 8);
-    if ((s = ensure(ctx, ctx->bins, std::min<size_t>(want_bins, 0xFFFFFFF0ull), "bins"))) return s;
+    const size_t want = std::min<size_t>(std::max<size_t>((size_t)4 << 20, (size_t)n_tris * 4), 0xFFFFFFF0ull);
+    for (DevArray<uint32_t> *a : {&ctx->pair_prim, &ctx->pair_tile, &ctx->pair_rank, &ctx->bins})
+        if ((s = ensure(ctx, *a, want, "bins"))) return s;
 
     for (int attempt = 0;; attempt++) {
-        p.clip = ctx->clip.ptr, p.vary = ctx->vary.ptr, p.recs = ctx->recs.ptr;
-        p.nrec = ctx->counters, p.flags = ctx->counters + 1;
-        p.tile_count = ctx->tile_count.ptr, p.tile_off = ctx->tile_off.ptr, p.tile_cursor = ctx->tile_cursor.ptr;
-        p.bins = ctx->bins.ptr, p.bin_cap = (uint32_t)std::min<size_t>(ctx->bins.cap, 0xFFFFFFF0ull);
-        if ((s = timed(ctx, "vertex", [&] { return launch_vertex(p, ctx->stream); }))) return s;
+        p.recs = ctx->recs.ptr;
+        p.ctr = ctx->counters;
+        p.tile_count = ctx->tile_count.ptr, p.tile_off = ctx->tile_off.ptr;
+        p.pair_prim = ctx->pair_prim.ptr, p.pair_tile = ctx->pair_tile.ptr, p.pair_rank = ctx->pair_rank.ptr;
+        p.bins = ctx->bins.ptr;
+        p.bin_cap = (uint32_t)std::min<size_t>(
+            std::min(std::min(ctx->bins.cap, ctx->pair_prim.cap), std::min(ctx->pair_tile.cap, ctx->pair_rank.cap)),
+            0xFFFFFFF0ull);
         if ((s = timed(ctx, "setup", [&] { return launch_setup(p, ctx->stream); }))) return s;
-        if ((s = timed(ctx, "scan", [&] { return launch_scan(p, ctx->stream); }))) return s;
         if ((s = timed(ctx, "scatter", [&] { return launch_scatter(p, ctx->stream); }))) return s;
         if ((s = timed(ctx, "raster", [&] { return launch_raster(p, ctx->stream); }))) return s;
         if (!sync) return EUCHIP_OK;
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->host_flags, ctx->counters, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->host_counters, ctx->counters, sizeof(Counters), hipMemcpyDeviceToHost,
+                                    ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->profiling) resolve_pending(ctx);
-        if (!(ctx->host_flags[1] & FLAG_BIN_OVERFLOW)) return EUCHIP_OK;
+        if (!(ctx->host_counters->flags & FLAG_BIN_OVERFLOW)) return EUCHIP_OK;
         // Bin storage overflowed: nothing was written.  Reset and retry larger.
-        HIP_TRY(ctx, hipMemsetAsync(ctx->counters, 0, 8, ctx->stream));
-        if (attempt >= 4 || ctx->bins.cap >= 0xFFFFFFF0ull)
-            return fail(ctx, EUCHIP_OVER_BUDGET, "bin storage overflow");
-        if ((s = ensure(ctx, ctx->bins, std::min<size_t>(ctx->bins.cap * 4, 0xFFFFFFF0ull), "bins"))) return s;
+        HIP_TRY(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(Counters), ctx->stream));
+        const size_t need = (size_t)ctx->host_counters->npairs;
+        if (attempt >= 3 || need >= 0xFFFFFFF0ull) return fail(ctx, EUCHIP_OVER_BUDGET, "bin storage overflow");
+        for (DevArray<uint32_t> *a : {&ctx->pair_prim, &ctx->pair_tile, &ctx->pair_rank, &ctx->bins})
+            if ((s = ensure(ctx, *a, std::min<size_t>(need + need / 4 + 1024, 0xFFFFFFF0ull), "bins"))) return s;
     }
 }
 
@@ -347,9 +347,9 @@ euchip_status euchip_ctx_create(int device, euchip_ctx **out) {
     euchip_ctx *ctx = new euchip_ctx();
     ctx->device = device;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc((void **)&ctx->counters, 64) != hipSuccess ||
-        hipHostMalloc((void **)&ctx->host_flags, 64, hipHostMallocDefault) != hipSuccess ||
-        hipMemset(ctx->counters, 0, 64) != hipSuccess) {
+        hipMalloc((void **)&ctx->counters, sizeof(Counters)) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->host_counters, sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
+        hipMemset(ctx->counters, 0, sizeof(Counters)) != hipSuccess) {
         euchip_ctx_destroy(ctx);
         return EUCHIP_HIP_ERROR;
     }
@@ -366,15 +366,15 @@ euchip_status euchip_ctx_destroy(euchip_ctx *ctx) {
         (void)hipEventDestroy(p.b);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-    (void)hipFree(ctx->clip.ptr);
-    (void)hipFree(ctx->vary.ptr);
     (void)hipFree(ctx->recs.ptr);
     (void)hipFree(ctx->tile_count.ptr);
     (void)hipFree(ctx->tile_off.ptr);
-    (void)hipFree(ctx->tile_cursor.ptr);
+    (void)hipFree(ctx->pair_prim.ptr);
+    (void)hipFree(ctx->pair_tile.ptr);
+    (void)hipFree(ctx->pair_rank.ptr);
     (void)hipFree(ctx->bins.ptr);
     (void)hipFree(ctx->counters);
-    if (ctx->host_flags) (void)hipHostFree(ctx->host_flags);
+    if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return EUCHIP_OK;
@@ -386,11 +386,12 @@ void *euchip_ctx_stream(euchip_ctx *ctx) { return ctx ? (void *)ctx->stream : nu
 
 euchip_status euchip_synchronize(euchip_ctx *ctx) {
     if (!ctx) return EUCHIP_INVALID_ARG;
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->host_flags, ctx->counters, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->host_counters, ctx->counters, sizeof(Counters), hipMemcpyDeviceToHost,
+                                ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->profiling) resolve_pending(ctx);
-    if (ctx->host_flags[1] & FLAG_BIN_OVERFLOW) {
-        HIP_TRY(ctx, hipMemset(ctx->counters, 0, 8));
+    if (ctx->host_counters->flags & FLAG_BIN_OVERFLOW) {
+        HIP_TRY(ctx, hipMemset(ctx->counters, 0, sizeof(Counters)));
         return fail(ctx, EUCHIP_OVER_BUDGET, "bin storage overflow in an asynchronous frame; it was dropped");
     }
     return EUCHIP_OK;

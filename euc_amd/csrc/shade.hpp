@@ -3,7 +3,9 @@
 // Every function here is evaluated with -ffp-contract=off and IEEE f32
 // division/sqrt, so each operation rounds once in the order written.  The
 // order follows the reference's expressions (triangles.rs, math.rs,
-// sampler/linear.rs) and this project's fixed-shader definitions.
+// sampler/linear.rs) and this project's fixed-shader definitions.  Shaders are
+// template parameters so that VertexData arrays have compile-time sizes and
+// live in registers.
 #pragma once
 #include "internal.hpp"
 
@@ -80,7 +82,7 @@ EUCHIP_DEV void lighting2(const float *u, const float wpos[3], const float nrm[3
 }
 
 // Bilinear sample, Clamped<Linear> (sampler/mod.rs:183-186, sampler/linear.rs:30-66)
-// of an RGBA8 texture mapped to f32 per channel, or of an f32 map (nc = 1).
+// of an RGBA8 texture mapped to f32 per channel (NC = 4), or of an f32 map (NC = 1).
 template <int NC>
 EUCHIP_DEV void sample_bilinear(const void *data, uint32_t w, uint32_t h, float x, float y, bool clamped,
                                 float *out) {
@@ -119,6 +121,75 @@ EUCHIP_DEV void sample_bilinear(const void *data, uint32_t w, uint32_t h, float 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pipeline::vertex of the fixed shader set
+// ---------------------------------------------------------------------------
+
+// World position (model * (p, 1)) or, for CLIP input, the position itself.
+EUCHIP_DEV void vertex_world(const FrameParams &p, uint32_t inst, uint32_t vi, float wpos[4]) {
+    const float *P = p.pos + (size_t)p.pos_comps * vi;
+    if (p.xform == 0) {
+        wpos[0] = P[0], wpos[1] = P[1], wpos[2] = P[2];
+        wpos[3] = p.pos_comps == 4 ? P[3] : 1.0f;
+    } else {
+        const float p4[4] = {P[0], P[1], P[2], 1.0f};
+        mat4_vec4(p.models + 16u * inst, p4, wpos);
+    }
+}
+
+// Homogeneous clip position returned by vertex().
+EUCHIP_DEV void vertex_clip(const FrameParams &p, uint32_t inst, uint32_t vi, float clip[4]) {
+    float w[4];
+    vertex_world(p, inst, vi, w);
+    if (p.xform == 0) {
+        clip[0] = w[0], clip[1] = w[1], clip[2] = w[2], clip[3] = w[3];
+    } else {
+        mat4_vec4(p.u + U_VP, w, clip);
+    }
+}
+
+// VertexData returned by vertex().
+template <uint32_t SH>
+EUCHIP_DEV void vertex_data(const FrameParams &p, uint32_t inst, uint32_t vi, float *vd) {
+    if constexpr (SH == FLAT) {
+        vd[0] = __uint_as_float(p.rgba8[vi]);
+    } else if constexpr (SH == COLOR) {
+        const float *c = p.col + 4u * vi;
+        vd[0] = c[0], vd[1] = c[1], vd[2] = c[2], vd[3] = c[3];
+    } else if constexpr (SH == TEXTURED) {
+        vd[0] = p.uv[2u * vi], vd[1] = p.uv[2u * vi + 1];
+    } else if constexpr (SH == DEPTH_ONLY) {
+        (void)vd;
+    } else {
+        float wpos[4], wn[4];
+        vertex_world(p, inst, vi, wpos);
+        const float *N = p.nrm + 3u * vi;
+        float n4[4] = {N[0], N[1], N[2], 0.0f};
+        if constexpr (SH == PHONG_SHADOW) n4[0] = -N[0], n4[1] = -N[1], n4[2] = -N[2];
+        if (p.xform == 0) wn[0] = n4[0], wn[1] = n4[1], wn[2] = n4[2], wn[3] = n4[3];
+        else mat4_vec4(p.models + 16u * inst, n4, wn);
+        if constexpr (SH == GOURAUD) {
+            lighting2(p.u, wpos, wn, vd);
+        } else {
+            vd[0] = wpos[0], vd[1] = wpos[1], vd[2] = wpos[2];
+            vd[3] = wn[0], vd[4] = wn[1], vd[5] = wn[2];
+            if constexpr (SH == PHONG_SHADOW || SH == TEX_SHADOW) {
+                const float w1[4] = {wpos[0], wpos[1], wpos[2], 1.0f};
+                float l4[4];
+                mat4_vec4(p.u + U_LVP, w1, l4);
+                int o = 6;
+                if constexpr (SH == TEX_SHADOW) {
+                    vd[6] = p.uv[2u * vi], vd[7] = p.uv[2u * vi + 1];
+                    o = 8;
+                }
+                vd[o + 0] = l4[0] / l4[3];
+                vd[o + 1] = l4[1] / l4[3];
+                vd[o + 2] = l4[2] / l4[3];
+            }
+        }
+    }
+}
+
 // Shadow-map lookup of examples/teapot.rs:123-128.
 EUCHIP_DEV bool in_light(const FrameParams &p, const float lvp[3]) {
     float sx = (lvp[0] * 1.0f) * 0.5f + 0.5f;
@@ -129,20 +200,20 @@ EUCHIP_DEV bool in_light(const FrameParams &p, const float lvp[3]) {
     return lvp[2] < ld;
 }
 
-// Pipeline::fragment of the fixed shader set.  vd: VertexData; out: Rgba<f32>
-// (FLAT: colour bits in out[0]).
+// Pipeline::fragment of the fixed shader set.  out: Rgba<f32> (FLAT: colour bits in out[0]).
+template <uint32_t SH>
 EUCHIP_DEV void fragment_shader(const FrameParams &p, const float *vd, float out[4]) {
     const float *u = p.u;
     out[0] = out[1] = out[2] = out[3] = 0.0f;
-    switch (p.shader) {
-    case FLAT: out[0] = vd[0]; break;
-    case COLOR:
-    case GOURAUD:
+    if constexpr (SH == FLAT) {
+        out[0] = vd[0];
+    } else if constexpr (SH == COLOR || SH == GOURAUD) {
         out[0] = vd[0], out[1] = vd[1], out[2] = vd[2], out[3] = vd[3];
-        break;
-    case PHONG: lighting2(u, vd, vd + 3, out); break;
-    case TEXTURED: sample_bilinear<4>(p.tex, p.tex_w, p.tex_h, vd[0], vd[1], false, out); break;
-    case PHONG_SHADOW: {
+    } else if constexpr (SH == PHONG) {
+        lighting2(u, vd, vd + 3, out);
+    } else if constexpr (SH == TEXTURED) {
+        sample_bilinear<4>(p.tex, p.tex_w, p.tex_h, vd[0], vd[1], false, out);
+    } else if constexpr (SH == PHONG_SHADOW) {
         float wn[3], cd[3], ld[3], t3[3];
         normalize3(vd + 3, wn);
         for (int c = 0; c < 3; c++) t3[c] = u[U_CAM + c] - vd[c];
@@ -157,9 +228,7 @@ EUCHIP_DEV void fragment_shader(const FrameParams &p, const float *vd, float out
         float specular = pow30(rmax(dot3s(r, ncd), 0.0f)) * u[U_K + 2];
         float light = u[U_K + 0] + (in_light(p, vd + 6) ? diffuse + specular : 0.0f);
         for (int c = 0; c < 4; c++) out[c] = u[U_SURF + c] * light;
-        break;
-    }
-    case TEX_SHADOW: {
+    } else if constexpr (SH == TEX_SHADOW) {
         float n[3], texc[4];
         normalize3(vd + 3, n);
         float ndl = rmax(dot3s(n, u + U_L0), 0.0f);
@@ -170,18 +239,16 @@ EUCHIP_DEV void fragment_shader(const FrameParams &p, const float *vd, float out
         out[1] = texc[1] * s;
         out[2] = texc[2] * s;
         out[3] = texc[3] * (1.0f / 255.0f);
-        break;
-    }
-    default: break;
     }
 }
 
-// Pipeline::blend: every fixed shader replaces the old pixel.
-EUCHIP_DEV uint32_t blend_shader(const FrameParams &p, uint32_t old, const float f[4]) {
-    if (p.shader == FLAT) return __float_as_uint(f[0]);
-    if (p.shader == DEPTH_ONLY) return old;
+// Pipeline::blend: every fixed shader replaces the old pixel (DEPTH_ONLY's Unit keeps it).
+template <uint32_t SH>
+EUCHIP_DEV uint32_t blend_shader(uint32_t old, const float f[4]) {
+    if constexpr (SH == FLAT) return __float_as_uint(f[0]);
+    if constexpr (SH == DEPTH_ONLY) return old;
     uint32_t b[4];
-    if (p.shader == TEXTURED) {
+    if constexpr (SH == TEXTURED) {
 #pragma unroll
         for (int c = 0; c < 4; c++) b[c] = sat_u8(f[c]);
     } else {

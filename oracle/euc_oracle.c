@@ -804,6 +804,7 @@ static int render_impl(const orc_desc *d, const float *u, const orc_stream *s, c
         render_inner(&j, 0, tgt[1]);
     } else {
         size_t group_rows = (size_t)20000 * ((size_t)1 << msaa_level) / (tgt[0] > 1 ? tgt[0] : 1);
+        if (d->group_rows > 0) group_rows = (size_t)d->group_rows;
         if (group_rows == 0) {
             rc = ORC_BAND_ZERO;
         } else {

@@ -57,6 +57,7 @@ typedef struct {
     int32_t aa_level;       /* AaMode::Msaa { level }                               */
     int32_t band_policy;    /* ORC_BANDS_*                                          */
     int32_t threads;        /* worker threads for PAR (available_parallelism)       */
+    int32_t group_rows;     /* tests only: PAR band height override (0 = 20000*2^L/W) */
 } orc_desc;
 
 /* Vertex stream: element e -> (instance e / count, vertex idx[e % count]). */

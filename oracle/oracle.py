@@ -33,7 +33,7 @@ class OrcDesc(C.Structure):
         ("depth_test", C.c_int32), ("depth_write", C.c_int32), ("y_up", C.c_int32),
         ("z_clip", C.c_int32), ("z_min", C.c_float), ("z_max", C.c_float),
         ("cull", C.c_int32), ("aa_msaa", C.c_int32), ("aa_level", C.c_int32),
-        ("band_policy", C.c_int32), ("threads", C.c_int32),
+        ("band_policy", C.c_int32), ("threads", C.c_int32), ("group_rows", C.c_int32),
     ]
 
 
@@ -91,7 +91,8 @@ def make_desc(p) -> OrcDesc:
         shader=p.shader, xform=p.xform, pixel_write=int(p.pixel_write), depth_test=p.depth_test,
         depth_write=int(p.depth_write), y_up=int(p.y_up), z_clip=int(p.z_clip),
         z_min=float(p.z_min), z_max=float(p.z_max), cull=p.cull, aa_msaa=int(p.aa_msaa),
-        aa_level=int(p.aa_level), band_policy=p.band_policy, threads=int(p.threads))
+        aa_level=int(p.aa_level), band_policy=p.band_policy, threads=int(p.threads),
+        group_rows=int(getattr(p, "band_rows_override", 0)))
 
 
 def _texture(t):
