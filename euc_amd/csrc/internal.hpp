@@ -28,29 +28,29 @@ constexpr __host__ __device__ uint32_t vd_count(uint32_t shader) {
          : shader == TEX_SHADOW ? 11 : 0;
 }
 
-// One surviving triangle after setup (triangles.rs:55-171), stored at its
-// primitive index; 128 bytes = 8 dwordx4.
-struct alignas(16) TriRec {
+// One triangle after setup (triangles.rs:55-171).  Lives in registers: the
+// raster pass recomputes it from the vertices' clip positions instead of
+// storing it.
+struct TriRec {
     float o[3], dx[3], dy[3];   // w_hom_origin, w_hom_dx, w_hom_dy
     float hz[3];                // verts_hom[i][2] after flip/reversal
     float vby[6];               // verts_by_y (screen x, y) x 3
     uint32_t xmin, xmax;        // bounds_clamped x, clamped to [0, W]
     uint32_t ymin, ymax;        // bounds_clamped y, clamped to [0, H]
     uint32_t flags;             // bit 0: all three vertices pass the z clip
-    uint32_t vi[3];             // vertex index of each (post-reversal) vertex
-    uint32_t inst[3];           // instance of each (post-reversal) vertex
-    uint32_t pad[3];
+    uint32_t sid[3];            // shaded-vertex id (instance * n_vertices + vertex), post-reversal
 };
-static_assert(sizeof(TriRec) == 128, "TriRec must stay 128 bytes");
 
 enum FrameFlags : uint32_t { FLAG_BIN_OVERFLOW = 1u };
 
-// Device counters of one context (zero between frames).
+// Device counters of one context.  ovf[f & 1] counts the overflow entries of
+// frame f; the raster pass of frame f zeroes ovf[(f + 1) & 1] for the next one.
+// Each counter sits on its own 128-byte line.
 struct Counters {
-    uint32_t npairs;     // (triangle, tile) pairs emitted by k_setup
-    uint32_t flags;      // FrameFlags
-    uint32_t done;       // k_setup workgroups finished (last one scans)
-    uint32_t pad[13];
+    uint32_t ovf[2][32];
+    uint32_t flags;      // FrameFlags, sticky until the host clears them
+    uint32_t max_count;  // largest per-tile triangle count seen (host resizes bins)
+    uint32_t pad[30];
 };
 
 struct FrameParams {
@@ -80,21 +80,22 @@ struct FrameParams {
     const float *shadow;
     uint32_t sh_w, sh_h;
     // per-frame scratch (device)
-    TriRec *recs;          // indexed by primitive
+    float4 *clip;          // clip position per shaded vertex (instance * n_vertices + vertex)
+    float *vary;           // VertexData per shaded vertex (instance * n_vertices + vertex)
     Counters *ctr;
     uint32_t *tile_count;  // per tile, zero between frames
-    uint32_t *tile_off;    // ntiles + 1
-    uint32_t *pair_prim, *pair_tile, *pair_rank;
-    uint32_t *bins;
-    uint32_t bin_cap;
+    uint32_t *bins;        // tile * tile_cap + rank -> primitive
+    uint32_t tile_cap;     // bin slots per tile
+    uint2 *ovf;            // (tile, primitive) of ranks >= tile_cap
+    uint32_t ovf_cap;
+    uint32_t frame;        // frame parity selects the overflow counter
     uint32_t ntx, nty;
     float u[kUniforms];
 };
 
 // Kernel launchers (kernels.hip).  All enqueue on `s`.
 hipError_t launch_setup(const FrameParams &p, hipStream_t s);
-hipError_t launch_scatter(const FrameParams &p, hipStream_t s);
-hipError_t launch_raster(const FrameParams &p, hipStream_t s);
+hipError_t launch_raster(const FrameParams &p, hipStream_t s, uint32_t max_groups);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s);
 
 }  // namespace euchip

@@ -1,26 +1,28 @@
 // kernels.hip -- gfx950 kernels of the triangle path.
 //
-// Frame = setup -> scatter -> raster (three launches):
+// Frame = setup -> raster (two launches):
 //   k_setup   one lane per triangle: vertex positions (the fixed vertex shader),
 //             euc's triangle setup (triangles.rs:55-171) in IEEE f32, culling,
-//             a 128-byte record at the primitive's index, and one
-//             (primitive, tile, rank) pair per covered 32x8 tile, the rank
-//             taken from a per-tile atomic counter.  The last workgroup to
-//             finish turns the per-tile counts into bin offsets.
-//   k_scatter one lane per pair: bins[offset[tile] + rank] = primitive.
-//   k_raster  one 256-thread workgroup per 32x8 tile, one pixel per thread.
-//             Each pixel keeps a 64-bit key in LDS that encodes the winning
-//             fragment; the depth rule becomes an order-independent reduction
-//             (atomic min/max of the key) because every fixed shader's blend
-//             replaces the old pixel: the winner is a function of the set of
-//             fragments, with primitive order as the tie-break (see
-//             init_key).  Triangles that touch few pixels of the tile are
-//             rasterised one per lane, walking their rows exactly as
+//             a 128-byte record at the primitive's index, and the primitive
+//             appended to the bin of every 32x8 tile its bounding box covers
+//             (slot tile * cap + rank, rank from a per-tile atomic counter;
+//             ranks past the capacity go to a shared overflow list).  The same
+//             launch runs the vertex shader's VertexData once per shaded vertex.
+//   k_raster  256-thread workgroups that loop over tiles (XCD-contiguous), one
+//             pixel per thread.  Each pixel keeps a 64-bit key in LDS that
+//             encodes the winning fragment; the depth rule becomes an
+//             order-independent reduction (atomic min/max of the key) because
+//             every fixed shader's blend replaces the old pixel: the winner is a
+//             function of the set of fragments, with primitive order as the
+//             tie-break (see init_key).  Triangles that touch few pixels of the
+//             tile are rasterised one per lane, walking their rows exactly as
 //             triangles.rs:219-303 does; larger ones are tested pixel-parallel.
 //             The winner is then shaded once and colour and depth are written
 //             once, in full 128-byte rows.
 #include "internal.hpp"
 #include "shade.hpp"
+
+#include <algorithm>
 
 namespace euchip {
 
@@ -76,239 +78,225 @@ EUCHIP_DEV void for_each_tile(bool act, uint32_t tx0, uint32_t ty0, uint32_t nx,
     }
 }
 
-// Bin offsets from per-tile counts, run by the last k_setup workgroup (256
-// threads).  Counts are read with device-scope atomic loads (they were
-// produced by device-scope atomics of every workgroup) and reset to zero for
-// the next frame.
-EUCHIP_DEV void scan_tile_counts(const FrameParams &p) {
-    __shared__ uint32_t buf[4096];
-    __shared__ uint32_t part[256];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t n = p.ntx * p.nty;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < n; base += 4096u) {
-        const uint32_t m = n - base < 4096u ? n - base : 4096u;
-        for (uint32_t i = tid; i < m; i += 256u) {
-            buf[i] = __hip_atomic_load(p.tile_count + base + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            p.tile_count[base + i] = 0u;
-        }
-        __syncthreads();
-        const uint32_t per = (m + 255u) / 256u;
-        const uint32_t b0 = tid * per < m ? tid * per : m;
-        const uint32_t b1 = b0 + per < m ? b0 + per : m;
-        uint32_t s = 0;
-        for (uint32_t i = b0; i < b1; i++) s += buf[i];
-        part[tid] = s;
-        __syncthreads();
-        for (uint32_t d = 1; d < 256u; d <<= 1) {
-            uint32_t v = tid >= d ? part[tid - d] : 0u;
-            __syncthreads();
-            part[tid] += v;
-            __syncthreads();
-        }
-        uint32_t run = carry + part[tid] - s;
-        for (uint32_t i = b0; i < b1; i++) {
-            uint32_t c = buf[i];
-            buf[i] = run;
-            run += c;
-        }
-        const uint32_t total = part[255];
-        __syncthreads();
-        for (uint32_t i = tid; i < m; i += 256u) p.tile_off[base + i] = buf[i];
-        carry += total;
-        __syncthreads();
-    }
-    if (tid == 0) {
-        p.tile_off[n] = carry;
-        const uint32_t np = __hip_atomic_load(&p.ctr->npairs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (np > p.bin_cap || carry != np) atomicOr(&p.ctr->flags, FLAG_BIN_OVERFLOW);
-        p.ctr->done = 0u;
-    }
-}
-
 // ---------------------------------------------------------------------------
-// triangle setup (triangles.rs:55-171), one lane per triangle
+// triangle setup (triangles.rs:55-171) of primitive t, whole-target band clamp.
+// CLIPBUF: read the vertices' clip positions from p.clip (written by k_setup)
+// instead of running the vertex shader.  Returns false when the triangle is
+// culled or its bounding box is empty.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_setup(FrameParams p) {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    bool keep = false;
-    TriRec R;
-    if (t < p.n_tris) {
-        float hom[3][4], euc[3][3];
-        uint32_t vis[3], ins[3];
-        const float flip1 = p.y_up ? -1.0f : 1.0f;
+template <bool CLIPBUF>
+EUCHIP_DEV bool tri_setup(const FrameParams &p, uint32_t t, TriRec &R) {
+    float hom[3][4], euc[3][3];
+    uint32_t sid[3];
+    const float flip1 = p.y_up ? -1.0f : 1.0f;
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const uint32_t e = 3u * t + (uint32_t)i;
-            const uint32_t inst = e / p.count;
-            const uint32_t k = e - inst * p.count;
-            const uint32_t vi = p.idx ? p.idx[k] : k;
-            vis[i] = vi;
-            ins[i] = inst;
-            float c[4];
+    for (int i = 0; i < 3; i++) {
+        const uint32_t e = 3u * t + (uint32_t)i;
+        const uint32_t inst = e / p.count;
+        const uint32_t k = e - inst * p.count;
+        const uint32_t vi = p.idx ? p.idx[k] : k;
+        sid[i] = inst * p.n_vertices + vi;
+        float c[4];
+        if (CLIPBUF) {
+            const float4 q = p.clip[sid[i]];
+            c[0] = q.x, c[1] = q.y, c[2] = q.z, c[3] = q.w;
+        } else {
             vertex_clip(p, inst, vi, c);
-            hom[i][0] = c[0] * 1.0f;
-            hom[i][1] = c[1] * flip1;
-            hom[i][2] = c[2];
-            hom[i][3] = c[3];
         }
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            euc[i][0] = hom[i][0] / hom[i][3];
-            euc[i][1] = hom[i][1] / hom[i][3];
-            euc[i][2] = hom[i][2] / hom[i][3];
-        }
-        // winding = cross(e1 - e0, e2 - e0).z
-        const float a0 = euc[1][0] - euc[0][0], a1 = euc[1][1] - euc[0][1];
-        const float b0 = euc[2][0] - euc[0][0], b1 = euc[2][1] - euc[0][1];
-        const float winding = a0 * b1 - a1 * b0;
-        bool culled = false;
-        if (p.cull == 1u) culled = winding * 1.0f < 0.0f;
-        else if (p.cull == 2u) culled = winding * -1.0f < 0.0f;
-        if (!culled) {
-            if (winding >= 0.0f) {
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    float tmp = hom[0][c];
-                    hom[0][c] = hom[2][c];
-                    hom[2][c] = tmp;
-                }
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    float tmp = euc[0][c];
-                    euc[0][c] = euc[2][c];
-                    euc[2][c] = tmp;
-                }
-                uint32_t ts = vis[0];
-                vis[0] = vis[2];
-                vis[2] = ts;
-                ts = ins[0];
-                ins[0] = ins[2];
-                ins[2] = ts;
-            }
-            const float sx = (float)p.W, sy = (float)p.H;
-            const float T[3][3] = {{p.ndc_x, 0.0f, -1.0f}, {0.0f, p.ndc_y, 1.0f}, {0.0f, 0.0f, 1.0f}};
-            // coords_to_weights (triangles.rs:86-102)
-            const float c3[3] = {hom[2][0], hom[2][1], hom[2][3]};
-            const float ca[3] = {hom[0][0] - c3[0], hom[0][1] - c3[1], hom[0][3] - c3[2]};
-            const float cb[3] = {hom[1][0] - c3[0], hom[1][1] - c3[1], hom[1][3] - c3[2]};
-            const float n[3] = {ca[1] * cb[2] - ca[2] * cb[1], ca[2] * cb[0] - ca[0] * cb[2],
-                                ca[0] * cb[1] - ca[1] * cb[0]};
-            const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
-            const float nc = n[0] * c3[0] + n[1] * c3[1] + n[2] * c3[2];
-            const float rd = nn > 0.0f ? 1.0f / rmin(nc, -__FLT_EPSILON__) : 1.0f;
-            float rows[3][3] = {{cb[1] * c3[2] - cb[2] * c3[1], cb[2] * c3[0] - cb[0] * c3[2],
-                                 cb[0] * c3[1] - cb[1] * c3[0]},
-                                {c3[1] * ca[2] - c3[2] * ca[1], c3[2] * ca[0] - c3[0] * ca[2],
-                                 c3[0] * ca[1] - c3[1] * ca[0]},
-                                {n[0], n[1], n[2]}};
-            float M[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-#pragma unroll
-                for (int j = 0; j < 3; j++) rows[i][j] = rows[i][j] * rd;
-#pragma unroll
-                for (int j = 0; j < 3; j++) M[i][j] = rows[i][0] * T[0][j] + rows[i][1] * T[1][j] + rows[i][2] * T[2][j];
-            }
-            // screen coordinates and bounds (triangles.rs:110-139); the band clamp of
-            // y is applied per row by k_raster, here the clamp is the whole target
-            float vs[3][2];
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                vs[i][0] = sx * (euc[i][0] * 0.5f + 0.5f);
-                vs[i][1] = sy * (euc[i][1] * -0.5f + 0.5f);
-            }
-            R.xmin = sat_clamp(rmin(rmin(vs[0][0], vs[1][0]), vs[2][0]) + 0.0f, 0u, p.W);
-            R.ymin = sat_clamp(rmin(rmin(vs[0][1], vs[1][1]), vs[2][1]) + 0.0f, 0u, p.H);
-            R.xmax = sat_clamp(rmax(rmax(vs[0][0], vs[1][0]), vs[2][0]) + 1.0f, 0u, p.W);
-            R.ymax = sat_clamp(rmax(rmax(vs[0][1], vs[1][1]), vs[2][1]) + 1.0f, 0u, p.H);
-            // w_hom at (0,0) and the per-pixel steps (triangles.rs:142-145)
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const float o = M[i][0] * 0.0f + M[i][1] * 0.0f + M[i][2] * 1.0f;
-                const float wx = M[i][0] * 1000.0f + M[i][1] * 0.0f + M[i][2] * 1.0f;
-                const float wy = M[i][0] * 0.0f + M[i][1] * 1000.0f + M[i][2] * 1.0f;
-                R.o[i] = o;
-                R.dx[i] = (wx - o) * (1.0f / 1000.0f);
-                R.dy[i] = (wy - o) * (1.0f / 1000.0f);
-                R.hz[i] = hom[i][2];
-                R.vi[i] = vis[i];
-                R.inst[i] = ins[i];
-            }
-            // verts_by_y (triangles.rs:148-171), ties resolved as the reference does
-            const float min_y = rmin(rmin(vs[0][1], vs[1][1]), vs[2][1]);
-            int ia, ib, ic;
-            if (vs[0][1] == min_y) {
-                if (vs[1][1] < vs[2][1]) ia = 0, ib = 1, ic = 2;
-                else ia = 0, ib = 2, ic = 1;
-            } else if (vs[1][1] == min_y) {
-                if (vs[0][1] < vs[2][1]) ia = 1, ib = 0, ic = 2;
-                else ia = 1, ib = 2, ic = 0;
-            } else {
-                if (vs[0][1] < vs[1][1]) ia = 2, ib = 0, ic = 1;
-                else ia = 2, ib = 1, ic = 0;
-            }
-            R.vby[0] = vs[ia][0], R.vby[1] = vs[ia][1];
-            R.vby[2] = vs[ib][0], R.vby[3] = vs[ib][1];
-            R.vby[4] = vs[ic][0], R.vby[5] = vs[ic][1];
-            const bool allin = passes_z_clip(p, euc[0][2]) && passes_z_clip(p, euc[1][2]) && passes_z_clip(p, euc[2][2]);
-            R.flags = allin ? 1u : 0u;
-            R.pad[0] = R.pad[1] = R.pad[2] = 0u;
-            keep = R.xmin < R.xmax && R.ymin < R.ymax;
-        }
+        hom[i][0] = c[0] * 1.0f;
+        hom[i][1] = c[1] * flip1;
+        hom[i][2] = c[2];
+        hom[i][3] = c[3];
     }
-    uint32_t tx0 = 0, ty0 = 0, nx = 0, ny = 0;
-    if (keep) {
-        uint4 *dst = reinterpret_cast<uint4 *>(p.recs + t);
-        const uint4 *src = reinterpret_cast<const uint4 *>(&R);
 #pragma unroll
-        for (int q = 0; q < 8; q++) dst[q] = src[q];
-        tx0 = R.xmin >> kTileShiftX;
-        ty0 = R.ymin >> kTileShiftY;
-        nx = ((R.xmax - 1u) >> kTileShiftX) - tx0 + 1u;
-        ny = ((R.ymax - 1u) >> kTileShiftY) - ty0 + 1u;
+    for (int i = 0; i < 3; i++) {
+        euc[i][0] = hom[i][0] / hom[i][3];
+        euc[i][1] = hom[i][1] / hom[i][3];
+        euc[i][2] = hom[i][2] / hom[i][3];
     }
-    // Pair slots: one wave-aggregated allocation, an exclusive prefix per lane.
-    const uint32_t n = keep ? nx * ny : 0u;
-    const uint32_t incl = wave_inclusive_sum(n);
-    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-    uint32_t base = 0;
-    if (total) {
-        if (lane_id() == 0u) base = atomicAdd(&p.ctr->npairs, total);
-        base = __builtin_amdgcn_readfirstlane(base);
-    }
-    const uint32_t pbase = base + incl - n;
-    uint32_t *cnt = p.tile_count;
-    const uint32_t cap = p.bin_cap;
-    for_each_tile(keep, tx0, ty0, nx, ny, p.ntx, t, pbase, [&](uint32_t tile, uint32_t j, uint32_t prim, uint32_t pb) {
-        const uint32_t rank = atomicAdd(cnt + tile, 1u);
-        const uint32_t i = pb + j;
-        if (i < cap) {
-            p.pair_prim[i] = prim;
-            p.pair_tile[i] = tile;
-            p.pair_rank[i] = rank;
+    // winding = cross(e1 - e0, e2 - e0).z
+    const float a0 = euc[1][0] - euc[0][0], a1 = euc[1][1] - euc[0][1];
+    const float b0 = euc[2][0] - euc[0][0], b1 = euc[2][1] - euc[0][1];
+    const float winding = a0 * b1 - a1 * b0;
+    if (p.cull == 1u && winding * 1.0f < 0.0f) return false;
+    if (p.cull == 2u && winding * -1.0f < 0.0f) return false;
+    if (winding >= 0.0f) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const float tmp = hom[0][c];
+            hom[0][c] = hom[2][c];
+            hom[2][c] = tmp;
         }
-    });
-    // The last workgroup to finish builds the bin offsets.
-    __shared__ uint32_t s_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(&p.ctr->done, 1u) == gridDim.x - 1u ? 1u : 0u;
-    __syncthreads();
-    if (s_last) {
-        __threadfence();
-        scan_tile_counts(p);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float tmp = euc[0][c];
+            euc[0][c] = euc[2][c];
+            euc[2][c] = tmp;
+        }
+        const uint32_t ts = sid[0];
+        sid[0] = sid[2];
+        sid[2] = ts;
+    }
+    const float sx = (float)p.W, sy = (float)p.H;
+    const float T[3][3] = {{p.ndc_x, 0.0f, -1.0f}, {0.0f, p.ndc_y, 1.0f}, {0.0f, 0.0f, 1.0f}};
+    // coords_to_weights (triangles.rs:86-102)
+    const float c3[3] = {hom[2][0], hom[2][1], hom[2][3]};
+    const float ca[3] = {hom[0][0] - c3[0], hom[0][1] - c3[1], hom[0][3] - c3[2]};
+    const float cb[3] = {hom[1][0] - c3[0], hom[1][1] - c3[1], hom[1][3] - c3[2]};
+    const float n[3] = {ca[1] * cb[2] - ca[2] * cb[1], ca[2] * cb[0] - ca[0] * cb[2], ca[0] * cb[1] - ca[1] * cb[0]};
+    const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    const float nc = n[0] * c3[0] + n[1] * c3[1] + n[2] * c3[2];
+    const float rd = nn > 0.0f ? 1.0f / rmin(nc, -__FLT_EPSILON__) : 1.0f;
+    float rows[3][3] = {{cb[1] * c3[2] - cb[2] * c3[1], cb[2] * c3[0] - cb[0] * c3[2], cb[0] * c3[1] - cb[1] * c3[0]},
+                        {c3[1] * ca[2] - c3[2] * ca[1], c3[2] * ca[0] - c3[0] * ca[2], c3[0] * ca[1] - c3[1] * ca[0]},
+                        {n[0], n[1], n[2]}};
+    float M[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) rows[i][j] = rows[i][j] * rd;
+#pragma unroll
+        for (int j = 0; j < 3; j++) M[i][j] = rows[i][0] * T[0][j] + rows[i][1] * T[1][j] + rows[i][2] * T[2][j];
+    }
+    // screen coordinates and bounds (triangles.rs:110-139); the band clamp of y
+    // is applied per row by the raster pass, here the clamp is the whole target
+    float vs[3][2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        vs[i][0] = sx * (euc[i][0] * 0.5f + 0.5f);
+        vs[i][1] = sy * (euc[i][1] * -0.5f + 0.5f);
+    }
+    R.xmin = sat_clamp(rmin(rmin(vs[0][0], vs[1][0]), vs[2][0]) + 0.0f, 0u, p.W);
+    R.ymin = sat_clamp(rmin(rmin(vs[0][1], vs[1][1]), vs[2][1]) + 0.0f, 0u, p.H);
+    R.xmax = sat_clamp(rmax(rmax(vs[0][0], vs[1][0]), vs[2][0]) + 1.0f, 0u, p.W);
+    R.ymax = sat_clamp(rmax(rmax(vs[0][1], vs[1][1]), vs[2][1]) + 1.0f, 0u, p.H);
+    // w_hom at (0,0) and the per-pixel steps (triangles.rs:142-145)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float o = M[i][0] * 0.0f + M[i][1] * 0.0f + M[i][2] * 1.0f;
+        const float wx = M[i][0] * 1000.0f + M[i][1] * 0.0f + M[i][2] * 1.0f;
+        const float wy = M[i][0] * 0.0f + M[i][1] * 1000.0f + M[i][2] * 1.0f;
+        R.o[i] = o;
+        R.dx[i] = (wx - o) * (1.0f / 1000.0f);
+        R.dy[i] = (wy - o) * (1.0f / 1000.0f);
+        R.hz[i] = hom[i][2];
+        R.sid[i] = sid[i];
+    }
+    // verts_by_y (triangles.rs:148-171), ties resolved as the reference does; kept in
+    // scalars (a runtime index into a local array would live in scratch memory)
+    const float x0 = vs[0][0], y0 = vs[0][1], x1 = vs[1][0], y1 = vs[1][1], x2 = vs[2][0], y2 = vs[2][1];
+    const float min_y = rmin(rmin(y0, y1), y2);
+    float ax, ay, bx, by, cx, cy;
+    if (y0 == min_y) {
+        ax = x0, ay = y0;
+        if (y1 < y2) bx = x1, by = y1, cx = x2, cy = y2;
+        else bx = x2, by = y2, cx = x1, cy = y1;
+    } else if (y1 == min_y) {
+        ax = x1, ay = y1;
+        if (y0 < y2) bx = x0, by = y0, cx = x2, cy = y2;
+        else bx = x2, by = y2, cx = x0, cy = y0;
+    } else {
+        ax = x2, ay = y2;
+        if (y0 < y1) bx = x0, by = y0, cx = x1, cy = y1;
+        else bx = x1, by = y1, cx = x0, cy = y0;
+    }
+    R.vby[0] = ax, R.vby[1] = ay, R.vby[2] = bx, R.vby[3] = by, R.vby[4] = cx, R.vby[5] = cy;
+    const bool allin = passes_z_clip(p, euc[0][2]) && passes_z_clip(p, euc[1][2]) && passes_z_clip(p, euc[2][2]);
+    R.flags = allin ? 1u : 0u;
+    return R.xmin < R.xmax && R.ymin < R.ymax;
+}
+
+// Bin append of one (tile, primitive) pair by global atomics (the fallback of
+// the LDS-grouped path).
+EUCHIP_DEV void bin_direct(const FrameParams &p, uint32_t tile, uint32_t prim) {
+    const uint32_t rank = atomicAdd(p.tile_count + tile, 1u);
+    if (rank < p.tile_cap) {
+        p.bins[(size_t)tile * p.tile_cap + rank] = prim;
+    } else {
+        const uint32_t o = atomicAdd(&p.ctr->ovf[p.frame & 1u][0], 1u);
+        if (o < p.ovf_cap) p.ovf[o] = make_uint2(tile, prim);
+        else atomicOr(&p.ctr->flags, FLAG_BIN_OVERFLOW);
     }
 }
 
+constexpr uint32_t kHash = 1024;   // LDS hash slots (distinct tiles) per setup workgroup
+constexpr uint32_t kPairs = 2048;  // (tile, primitive) pairs grouped in LDS per workgroup
+
 // ---------------------------------------------------------------------------
-// bins[offset[tile] + rank] = primitive (order inside a bin is irrelevant)
+// k_setup: per-vertex clip position + VertexData (thread t = shaded vertex t)
+// and per-triangle culling + binning (thread t = primitive t).
+// Binning groups the workgroup's pairs by tile in an LDS hash table, so each
+// distinct tile costs one global atomic per workgroup; ranks inside the
+// group come from LDS.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_scatter(FrameParams p) {
-    if (p.ctr->flags & FLAG_BIN_OVERFLOW) return;
-    const uint32_t n = p.ctr->npairs;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
-        p.bins[p.tile_off[p.pair_tile[i]] + p.pair_rank[i]] = p.pair_prim[i];
+template <uint32_t SH>
+__global__ __launch_bounds__(256) void k_setup(FrameParams p) {
+    __shared__ uint32_t hkey[kHash], hcnt[kHash], hbase[kHash];
+    __shared__ uint32_t pl_prim[kPairs], pl_slot[kPairs], pl_local[kPairs];
+    __shared__ uint32_t npl;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t t = blockIdx.x * 256u + tid;
+    for (uint32_t i = tid; i < kHash; i += 256u) hkey[i] = 0u, hcnt[i] = 0u;
+    if (tid == 0u) npl = 0u;
+    if (t < p.n_instances * p.n_vertices) {
+        const uint32_t inst = t / p.n_vertices;
+        const uint32_t vi = t - inst * p.n_vertices;
+        float c[4];
+        vertex_clip(p, inst, vi, c);
+        p.clip[t] = make_float4(c[0], c[1], c[2], c[3]);
+        if constexpr (vd_count(SH) > 0) {
+            float vd[VDN<SH>::N];
+            vertex_data<SH>(p, inst, vi, vd);
+#pragma unroll
+            for (uint32_t q = 0; q < vd_count(SH); q++) p.vary[(size_t)t * vd_count(SH) + q] = vd[q];
+        }
+    }
+    bool keep = false;
+    uint32_t tx0 = 0, ty0 = 0, nx = 0, ny = 0;
+    if (t < p.n_tris) {
+        TriRec R;
+        keep = tri_setup<false>(p, t, R);
+        if (keep) {
+            tx0 = R.xmin >> kTileShiftX;
+            ty0 = R.ymin >> kTileShiftY;
+            nx = ((R.xmax - 1u) >> kTileShiftX) - tx0 + 1u;
+            ny = ((R.ymax - 1u) >> kTileShiftY) - ty0 + 1u;
+        }
+    }
+    __syncthreads();
+    for_each_tile(keep, tx0, ty0, nx, ny, p.ntx, t, 0u, [&](uint32_t tile, uint32_t, uint32_t prim, uint32_t) {
+        uint32_t h = (tile * 2654435761u) >> 22;  // 10-bit multiplicative hash
+        for (uint32_t probe = 0; probe < 16u; probe++) {
+            const uint32_t old = atomicCAS(&hkey[h], 0u, tile + 1u);
+            if (old == 0u || old == tile + 1u) {
+                const uint32_t k = atomicAdd(&npl, 1u);
+                if (k < kPairs) {
+                    pl_prim[k] = prim;
+                    pl_slot[k] = h;
+                    pl_local[k] = atomicAdd(&hcnt[h], 1u);
+                    return;
+                }
+                break;
+            }
+            h = (h + 1u) & (kHash - 1u);
+        }
+        bin_direct(p, tile, prim);  // table or pair list full
+    });
+    __syncthreads();
+    for (uint32_t i = tid; i < kHash; i += 256u)
+        if (hcnt[i]) hbase[i] = atomicAdd(p.tile_count + (hkey[i] - 1u), hcnt[i]);
+    __syncthreads();
+    const uint32_t np = npl < kPairs ? npl : kPairs;
+    for (uint32_t k = tid; k < np; k += 256u) {
+        const uint32_t h = pl_slot[k], tile = hkey[h] - 1u, rank = hbase[h] + pl_local[k], prim = pl_prim[k];
+        if (rank < p.tile_cap) {
+            p.bins[(size_t)tile * p.tile_cap + rank] = prim;
+        } else {
+            const uint32_t o = atomicAdd(&p.ctr->ovf[p.frame & 1u][0], 1u);
+            if (o < p.ovf_cap) p.ovf[o] = make_uint2(tile, prim);
+            else atomicOr(&p.ctr->flags, FLAG_BIN_OVERFLOW);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -398,21 +386,8 @@ EUCHIP_DEV void band_of(const FrameParams &p, uint32_t y, uint32_t &by0, uint32_
     }
 }
 
-EUCHIP_DEV void load_rec(const TriRec *recs, uint32_t prim, TriRec &R) {
-    const uint4 *src = reinterpret_cast<const uint4 *>(recs + prim);
-    uint4 *dst = reinterpret_cast<uint4 *>(&R);
-#pragma unroll
-    for (int q = 0; q < 8; q++) dst[q] = src[q];
-}
-
 template <uint32_t SH>
-struct VD {
-    static constexpr uint32_t N = vd_count(SH) ? vd_count(SH) : 1;
-};
-
-// get_v_data (triangles.rs:274-294) and weighted_sum3 (math.rs:37-40)
-template <uint32_t SH>
-EUCHIP_DEV void v_data(const TriRec &R, const float (*V)[VD<SH>::N], float X, float Y, float *vd) {
+EUCHIP_DEV void v_data(const TriRec &R, const float (*V)[VDN<SH>::N], float X, float Y, float *vd) {
     float w[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) w[i] = (R.o[i] + R.dy[i] * Y) + R.dx[i] * X;
@@ -431,10 +406,14 @@ EUCHIP_DEV void v_data(const TriRec &R, const float (*V)[VD<SH>::N], float X, fl
 template <uint32_t SH>
 EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const TriRec &R, uint32_t px, uint32_t py, uint32_t by0,
                                 uint32_t old) {
-    constexpr uint32_t NV = VD<SH>::N;
+    constexpr uint32_t NV = VDN<SH>::N;
     float V[3][NV];
 #pragma unroll
-    for (int i = 0; i < 3; i++) vertex_data<SH>(p, R.inst[i], R.vi[i], V[i]);
+    for (int i = 0; i < 3; i++) {
+        const float *src = p.vary + (size_t)R.sid[i] * vd_count(SH);
+#pragma unroll
+        for (uint32_t c = 0; c < vd_count(SH); c++) V[i][c] = src[c];
+    }
     float frag[4], vd[NV];
     const uint32_t L = p.msaa_level;
     if (L == 0u) {
@@ -480,13 +459,13 @@ EUCHIP_DEV bool frag_z(const FrameParams &p, const TriRec &R, float w0, float w1
 // A lane rasterises one triangle over the rows it has in this tile, walking
 // each row from the row start exactly as the reference does.
 EUCHIP_DEV void raster_lane(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t X0t, uint32_t Y0t,
-                            uint32_t mode, uint64_t *key, const float *d0s) {
+                            uint32_t mode, uint64_t *key, const float *d0s, const uint32_t *band) {
     const uint32_t ry0 = R.ymin > Y0t ? R.ymin : Y0t;
     const uint32_t ry1 = R.ymax < Y0t + kTileH ? R.ymax : Y0t + kTileH;
     for (uint32_t y = ry0; y < ry1; y++) {
-        uint32_t by0, by1, x0, x1;
-        band_of(p, y, by0, by1);
-        row_span(R, y, by0, by1, x0, x1);
+        uint32_t x0, x1;
+        const uint32_t r = y - Y0t;
+        row_span(R, y, band[2 * r], band[2 * r + 1], x0, x1);
         const uint32_t xs = x0 > X0t ? x0 : X0t;
         const uint32_t xe = x1 < X0t + kTileW ? x1 : X0t + kTileW;
         if (xs >= xe) continue;
@@ -513,119 +492,155 @@ EUCHIP_DEV void raster_lane(const FrameParams &p, const TriRec &R, uint32_t prim
     }
 }
 
-// Workgroup-to-tile map: the hardware deals workgroups round-robin to the 8
-// XCDs; give each XCD a contiguous run of tiles so triangles shared by
-// neighbouring tiles stay in that XCD's L2.
-EUCHIP_DEV uint32_t xcd_tile(uint32_t b, uint32_t n) {
-    const uint32_t q = n >> 3, r = n & 7u;
-    const uint32_t x = b & 7u, i = b >> 3;
-    return x * q + (x < r ? x : r) + i;
-}
-
 // Lanes whose triangle would cost more than this many row steps in the tile
 // hand it to the pixel-parallel pass.
 constexpr uint32_t kLaneWork = 48;
 
+struct TileShared {
+    uint64_t key[kTileW * kTileH];   // winner key per pixel
+    float d0s[kTileW * kTileH];      // depth before the pass (the test's reference)
+    uint32_t big[256];               // triangles handed to the pixel-parallel pass
+    uint32_t band[2 * kTileH];       // [by0, by1) of the render_par band of each row
+    uint32_t nbig;
+};
+
+// One bin entry: rasterise it now (lane-parallel) or queue it for the
+// pixel-parallel pass.
+EUCHIP_DEV void bin_entry(const FrameParams &p, TileShared &sh, uint32_t prim, uint32_t X0t, uint32_t Y0t,
+                          uint32_t mode) {
+    TriRec R;
+    tri_setup<true>(p, prim, R);
+    const uint32_t ry0 = R.ymin > Y0t ? R.ymin : Y0t;
+    const uint32_t ry1 = R.ymax < Y0t + kTileH ? R.ymax : Y0t + kTileH;
+    const uint32_t xe = R.xmax < X0t + kTileW ? R.xmax : X0t + kTileW;
+    const uint32_t cost = (ry1 > ry0 ? ry1 - ry0 : 0u) * (xe > R.xmin ? xe - R.xmin : 0u);
+    if (cost <= kLaneWork) raster_lane(p, R, prim, X0t, Y0t, mode, sh.key, sh.d0s, sh.band);
+    else sh.big[atomicAdd(&sh.nbig, 1u)] = prim;
+}
+
+// The queued large triangles, one pixel per thread.
+EUCHIP_DEV void big_pass(const FrameParams &p, TileShared &sh, uint32_t tid, uint32_t px, uint32_t py, bool inside,
+                         uint32_t mode, float D0) {
+    const uint32_t nb = sh.nbig;
+    const uint32_t r = tid >> kTileShiftX;
+    for (uint32_t k = 0; k < nb; k++) {
+        const uint32_t prim = __builtin_amdgcn_readfirstlane(sh.big[k]);
+        TriRec R;
+        tri_setup<true>(p, prim, R);
+        if (!inside || py < R.ymin || py >= R.ymax) continue;
+        uint32_t x0, x1;
+        row_span(R, py, sh.band[2 * r], sh.band[2 * r + 1], x0, x1);
+        if (px < x0 || px >= x1) continue;
+        const float Y = (float)py, X0 = (float)x0;
+        float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
+        float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
+        float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
+        for (uint32_t x = x0; x < px; x++) {
+            w0 = w0 + R.dx[0];
+            w1 = w1 + R.dx[1];
+            w2 = w2 + R.dx[2];
+        }
+        float z;
+        if (frag_z(p, R, w0, w1, w2, z)) update_key(sh.key + tid, mode, p.depth_test, z, D0, prim);
+    }
+}
+
+// Range [beg, end) of this XCD's tiles: the hardware deals workgroups
+// round-robin to the 8 XCDs, so workgroup b runs on XCD b % 8; each XCD gets a
+// contiguous run of tiles so triangles shared by neighbouring tiles stay in
+// that XCD's L2.
+EUCHIP_DEV void xcd_range(uint32_t ntiles, uint32_t xcd, uint32_t &beg, uint32_t &end) {
+    const uint32_t q = ntiles >> 3, r = ntiles & 7u;
+    beg = xcd * q + (xcd < r ? xcd : r);
+    end = beg + q + (xcd < r ? 1u : 0u);
+}
+
 template <uint32_t SH>
 __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
-    __shared__ uint64_t key[kTileW * kTileH];
-    __shared__ float d0s[kTileW * kTileH];
-    __shared__ uint32_t big[256];
-    __shared__ uint32_t nbig;
+    __shared__ TileShared sh;
     if (p.ctr->flags & FLAG_BIN_OVERFLOW) return;
-    const uint32_t ntiles = p.ntx * p.nty;
-    const uint32_t tile = xcd_tile(blockIdx.x, ntiles);
     const uint32_t tid = threadIdx.x;
-    if (tile == 0u && tid == 0u) p.ctr->npairs = 0u;  // pair counter of the next frame
-    const uint32_t ty = tile / p.ntx, tx = tile - ty * p.ntx;
-    const uint32_t X0t = tx * kTileW, Y0t = ty * kTileH;
-    const uint32_t px = X0t + (tid & (kTileW - 1)), py = Y0t + (tid >> kTileShiftX);
-    const bool inside = px < p.W && py < p.H;
-    const size_t pidx = (size_t)py * p.W + px;
+    const uint32_t ntiles = p.ntx * p.nty;
+    if (blockIdx.x == 0u && tid == 0u) p.ctr->ovf[(p.frame + 1u) & 1u][0] = 0u;  // next frame's list
+    const uint32_t novf = p.ctr->ovf[p.frame & 1u][0] < p.ovf_cap ? p.ctr->ovf[p.frame & 1u][0] : p.ovf_cap;
     const bool uses_depth = p.depth_test != DT_NONE || p.depth_write;
-    float D0 = 0.0f;
-    if (inside && uses_depth) D0 = p.clear_depth_en ? p.clear_depth : p.depth[pidx];
     const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
                         : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
-    key[tid] = init_key(mode, D0);
-    d0s[tid] = D0;
-    if (tid == 0u) nbig = 0u;
-    __syncthreads();
-
-    const uint32_t beg = p.tile_off[tile], end = p.tile_off[tile + 1u];
-    for (uint32_t b = beg; b < end; b += 256u) {
-        const uint32_t i = b + tid;
-        if (i < end) {
-            const uint32_t prim = p.bins[i];
-            TriRec R;
-            load_rec(p.recs, prim, R);
-            const uint32_t ry0 = R.ymin > Y0t ? R.ymin : Y0t;
-            const uint32_t ry1 = R.ymax < Y0t + kTileH ? R.ymax : Y0t + kTileH;
-            const uint32_t xe = R.xmax < X0t + kTileW ? R.xmax : X0t + kTileW;
-            const uint32_t cost = (ry1 > ry0 ? ry1 - ry0 : 0u) * (xe > R.xmin ? xe - R.xmin : 0u);
-            if (cost <= kLaneWork) {
-                raster_lane(p, R, prim, X0t, Y0t, mode, key, d0s);
-            } else {
-                big[atomicAdd(&nbig, 1u)] = prim;
-            }
-        }
-        __syncthreads();
-        const uint32_t nb = nbig;
-        for (uint32_t k = 0; k < nb; k++) {
-            const uint32_t prim = __builtin_amdgcn_readfirstlane(big[k]);
-            TriRec R;
-            load_rec(p.recs, prim, R);
-            if (!inside || py < R.ymin || py >= R.ymax) continue;
-            uint32_t by0, by1, x0, x1;
-            band_of(p, py, by0, by1);
-            row_span(R, py, by0, by1, x0, x1);
-            if (px < x0 || px >= x1) continue;
-            const float Y = (float)py, X0 = (float)x0;
-            float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
-            float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
-            float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
-            for (uint32_t x = x0; x < px; x++) {
-                w0 = w0 + R.dx[0];
-                w1 = w1 + R.dx[1];
-                w2 = w2 + R.dx[2];
-            }
-            float z;
-            if (frag_z(p, R, w0, w1, w2, z)) update_key(key + tid, mode, p.depth_test, z, D0, prim);
-        }
-        __syncthreads();
-        if (tid == 0u) nbig = 0u;
-        __syncthreads();
-    }
-
-    if (!inside) return;
-    const uint64_t kk = key[tid];
-    bool has;
-    uint32_t prim;
-    float z;
-    if (mode == 0u) {
-        has = kk != 0ull;
-        prim = (uint32_t)(kk >> 32) - 1u;
-        z = __uint_as_float((uint32_t)kk);
-    } else {
-        has = ((uint32_t)kk) != 0u;
-        prim = ((uint32_t)kk >> 1) - 1u;
-        const uint32_t o = (uint32_t)(kk >> 32);
-        z = zunord(mode == 1u ? o : ~o, (uint32_t)kk & 1u);
-    }
-    if (p.pixel) {
-        if (p.pixel_write && has && SH != DEPTH_ONLY) {
+    uint32_t t0, t1;
+    xcd_range(ntiles, blockIdx.x & 7u, t0, t1);
+    const uint32_t per_xcd = gridDim.x >> 3;
+    for (uint32_t tile = t0 + (blockIdx.x >> 3); tile < t1; tile += per_xcd) {
+        const uint32_t ty = tile / p.ntx, tx = tile - ty * p.ntx;
+        const uint32_t X0t = tx * kTileW, Y0t = ty * kTileH;
+        const uint32_t px = X0t + (tid & (kTileW - 1)), py = Y0t + (tid >> kTileShiftX);
+        const bool inside = px < p.W && py < p.H;
+        const size_t pidx = (size_t)py * p.W + px;
+        float D0 = 0.0f;
+        if (inside && uses_depth) D0 = p.clear_depth_en ? p.clear_depth : p.depth[pidx];
+        sh.key[tid] = init_key(mode, D0);
+        sh.d0s[tid] = D0;
+        if (tid == 0u) sh.nbig = 0u;
+        if (tid < kTileH) {
             uint32_t by0, by1;
-            band_of(p, py, by0, by1);
-            TriRec R;
-            load_rec(p.recs, prim, R);
-            p.pixel[pidx] = shade_pixel<SH>(p, R, px, py, by0, p.clear_color_en ? p.clear_color : 0u);
-        } else if (p.clear_color_en) {
-            p.pixel[pidx] = p.clear_color;
+            band_of(p, Y0t + tid, by0, by1);
+            sh.band[2 * tid] = by0;
+            sh.band[2 * tid + 1] = by1;
         }
-    }
-    if (p.depth) {
-        if (has && p.depth_write) p.depth[pidx] = z;
-        else if (p.clear_depth_en) p.depth[pidx] = p.clear_depth;
+        const uint32_t cnt = p.tile_count[tile];
+        __syncthreads();
+        if (tid == 0u) {
+            p.tile_count[tile] = 0u;  // counter of the next frame
+            if (cnt > p.tile_cap) atomicMax(&p.ctr->max_count, cnt);
+        }
+        // this tile's bin, then (only when it overflowed) its entries of the overflow list
+        const uint32_t nb = cnt < p.tile_cap ? cnt : p.tile_cap;
+        const uint32_t *bin = p.bins + (size_t)tile * p.tile_cap;
+        const uint32_t nscan = nb + (cnt > p.tile_cap ? novf : 0u);
+        for (uint32_t b = 0; b < nscan; b += 256u) {
+            const uint32_t i = b + tid;
+            if (i < nb) {
+                bin_entry(p, sh, bin[i], X0t, Y0t, mode);
+            } else if (i < nscan) {
+                const uint2 e = p.ovf[i - nb];
+                if (e.x == tile) bin_entry(p, sh, e.y, X0t, Y0t, mode);
+            }
+            __syncthreads();
+            big_pass(p, sh, tid, px, py, inside, mode, D0);
+            __syncthreads();
+            if (tid == 0u) sh.nbig = 0u;
+            __syncthreads();
+        }
+        if (inside) {
+            const uint64_t kk = sh.key[tid];
+            bool has;
+            uint32_t prim;
+            float z;
+            if (mode == 0u) {
+                has = kk != 0ull;
+                prim = (uint32_t)(kk >> 32) - 1u;
+                z = __uint_as_float((uint32_t)kk);
+            } else {
+                has = ((uint32_t)kk) != 0u;
+                prim = ((uint32_t)kk >> 1) - 1u;
+                const uint32_t o = (uint32_t)(kk >> 32);
+                z = zunord(mode == 1u ? o : ~o, (uint32_t)kk & 1u);
+            }
+            if (p.pixel) {
+                if (p.pixel_write && has && SH != DEPTH_ONLY) {
+                    TriRec R;
+                    tri_setup<true>(p, prim, R);
+                    p.pixel[pidx] = shade_pixel<SH>(p, R, px, py, sh.band[2 * (tid >> kTileShiftX)],
+                                                    p.clear_color_en ? p.clear_color : 0u);
+                } else if (p.clear_color_en) {
+                    p.pixel[pidx] = p.clear_color;
+                }
+            }
+            if (p.depth) {
+                if (has && p.depth_write) p.depth[pidx] = z;
+                else if (p.clear_depth_en) p.depth[pidx] = p.clear_depth;
+            }
+        }
+        __syncthreads();  // LDS is reused by the next tile
     }
 }
 
@@ -639,32 +654,48 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *dst, uint32_t v, siz
 // ---------------------------------------------------------------------------
 static inline uint32_t blocks(size_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
-hipError_t launch_setup(const FrameParams &p, hipStream_t s) {
-    if (p.n_tris == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_setup, dim3(blocks(p.n_tris, 256)), dim3(256), 0, s, p);
-    return hipGetLastError();
+template <uint32_t SH>
+static void launch_setup_t(const FrameParams &p, hipStream_t s) {
+    const size_t n = std::max<size_t>(p.n_tris, (size_t)p.n_instances * p.n_vertices);
+    hipLaunchKernelGGL(k_setup<SH>, dim3(blocks(n, 256)), dim3(256), 0, s, p);
 }
 
-hipError_t launch_scatter(const FrameParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_scatter, dim3(1024), dim3(256), 0, s, p);
+hipError_t launch_setup(const FrameParams &p, hipStream_t s) {
+    switch (p.shader) {
+    case FLAT: launch_setup_t<FLAT>(p, s); break;
+    case COLOR: launch_setup_t<COLOR>(p, s); break;
+    case GOURAUD: launch_setup_t<GOURAUD>(p, s); break;
+    case PHONG: launch_setup_t<PHONG>(p, s); break;
+    case DEPTH_ONLY: launch_setup_t<DEPTH_ONLY>(p, s); break;
+    case TEXTURED: launch_setup_t<TEXTURED>(p, s); break;
+    case PHONG_SHADOW: launch_setup_t<PHONG_SHADOW>(p, s); break;
+    case TEX_SHADOW: launch_setup_t<TEX_SHADOW>(p, s); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 template <uint32_t SH>
-static void launch_raster_t(const FrameParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_raster<SH>, dim3(p.ntx * p.nty), dim3(256), 0, s, p);
+static void launch_raster_t(const FrameParams &p, hipStream_t s, uint32_t groups) {
+    hipLaunchKernelGGL(k_raster<SH>, dim3(groups), dim3(256), 0, s, p);
 }
 
-hipError_t launch_raster(const FrameParams &p, hipStream_t s) {
+// Workgroups of the tile loop: a multiple of the 8 XCDs, no more than the
+// tiles need (rounded up to 8) and no more than `max_groups`.
+hipError_t launch_raster(const FrameParams &p, hipStream_t s, uint32_t max_groups) {
+    const uint32_t ntiles = p.ntx * p.nty;
+    uint32_t g = ((ntiles + 7u) / 8u) * 8u;
+    const uint32_t cap = std::max<uint32_t>(8u, (max_groups / 8u) * 8u);
+    if (g > cap) g = cap;
     switch (p.shader) {
-    case FLAT: launch_raster_t<FLAT>(p, s); break;
-    case COLOR: launch_raster_t<COLOR>(p, s); break;
-    case GOURAUD: launch_raster_t<GOURAUD>(p, s); break;
-    case PHONG: launch_raster_t<PHONG>(p, s); break;
-    case DEPTH_ONLY: launch_raster_t<DEPTH_ONLY>(p, s); break;
-    case TEXTURED: launch_raster_t<TEXTURED>(p, s); break;
-    case PHONG_SHADOW: launch_raster_t<PHONG_SHADOW>(p, s); break;
-    case TEX_SHADOW: launch_raster_t<TEX_SHADOW>(p, s); break;
+    case FLAT: launch_raster_t<FLAT>(p, s, g); break;
+    case COLOR: launch_raster_t<COLOR>(p, s, g); break;
+    case GOURAUD: launch_raster_t<GOURAUD>(p, s, g); break;
+    case PHONG: launch_raster_t<PHONG>(p, s, g); break;
+    case DEPTH_ONLY: launch_raster_t<DEPTH_ONLY>(p, s, g); break;
+    case TEXTURED: launch_raster_t<TEXTURED>(p, s, g); break;
+    case PHONG_SHADOW: launch_raster_t<PHONG_SHADOW>(p, s, g); break;
+    case TEX_SHADOW: launch_raster_t<TEX_SHADOW>(p, s, g); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

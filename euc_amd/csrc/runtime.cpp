@@ -49,8 +49,13 @@ struct euchip_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, Stat> stats;
     // per-frame scratch, grown on demand
-    DevArray<TriRec> recs;
-    DevArray<uint32_t> tile_count, tile_off, pair_prim, pair_tile, pair_rank, bins;
+    DevArray<float4> clip;
+    DevArray<float> vary;
+    DevArray<uint32_t> tile_count, bins;
+    DevArray<uint2> ovf;
+    uint32_t tile_cap = 128;    // bin slots per tile; raised when a frame needed more
+    uint32_t frame = 0;
+    uint32_t max_groups = 2048; // resident raster workgroups: CUs x 8
     Counters *counters = nullptr;
     Counters *host_counters = nullptr;
 };
@@ -282,39 +287,44 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     const size_t ntiles = (size_t)p.ntx * p.nty;
 
     euchip_status s;
-    if ((s = ensure(ctx, ctx->recs, (size_t)n_tris, "triangle records"))) return s;
+    const size_t nsv = (size_t)mesh->n_instances * mesh->n_vertices;
+    if ((s = ensure(ctx, ctx->clip, std::max<size_t>(nsv, 1), "clip positions"))) return s;
+    if ((s = ensure(ctx, ctx->vary, std::max<size_t>(nsv * vd_count(sh), 1), "vertex data"))) return s;
     const bool fresh_tiles = ctx->tile_count.cap < ntiles;
     if ((s = ensure(ctx, ctx->tile_count, ntiles, "tile counts"))) return s;
     if (fresh_tiles) HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4, ctx->stream));
-    if ((s = ensure(ctx, ctx->tile_off, ntiles + 1, "tile offsets"))) return s;
-    const size_t want = std::min<size_t>(std::max<size_t>((size_t)4 << 20, (size_t)n_tris * 4), 0xFFFFFFF0ull);
-    for (DevArray<uint32_t> *a : {&ctx->pair_prim, &ctx->pair_tile, &ctx->pair_rank, &ctx->bins})
-        if ((s = ensure(ctx, *a, want, "bins"))) return s;
+    if ((s = ensure(ctx, ctx->ovf, std::max<size_t>((size_t)1 << 20, (size_t)n_tris), "overflow list"))) return s;
 
     for (int attempt = 0;; attempt++) {
-        p.recs = ctx->recs.ptr;
+        if ((s = ensure(ctx, ctx->bins, ntiles * ctx->tile_cap, "bins"))) return s;
+        p.clip = ctx->clip.ptr;
+        p.vary = ctx->vary.ptr;
         p.ctr = ctx->counters;
-        p.tile_count = ctx->tile_count.ptr, p.tile_off = ctx->tile_off.ptr;
-        p.pair_prim = ctx->pair_prim.ptr, p.pair_tile = ctx->pair_tile.ptr, p.pair_rank = ctx->pair_rank.ptr;
+        p.tile_count = ctx->tile_count.ptr;
         p.bins = ctx->bins.ptr;
-        p.bin_cap = (uint32_t)std::min<size_t>(
-            std::min(std::min(ctx->bins.cap, ctx->pair_prim.cap), std::min(ctx->pair_tile.cap, ctx->pair_rank.cap)),
-            0xFFFFFFF0ull);
+        p.tile_cap = ctx->tile_cap;
+        p.ovf = ctx->ovf.ptr;
+        p.ovf_cap = (uint32_t)std::min<size_t>(ctx->ovf.cap, 0xFFFFFFF0ull);
+        p.frame = ctx->frame++;
         if ((s = timed(ctx, "setup", [&] { return launch_setup(p, ctx->stream); }))) return s;
-        if ((s = timed(ctx, "scatter", [&] { return launch_scatter(p, ctx->stream); }))) return s;
-        if ((s = timed(ctx, "raster", [&] { return launch_raster(p, ctx->stream); }))) return s;
+        if ((s = timed(ctx, "raster", [&] { return launch_raster(p, ctx->stream, ctx->max_groups); }))) return s;
         if (!sync) return EUCHIP_OK;
         HIP_TRY(ctx, hipMemcpyAsync(ctx->host_counters, ctx->counters, sizeof(Counters), hipMemcpyDeviceToHost,
                                     ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->profiling) resolve_pending(ctx);
-        if (!(ctx->host_counters->flags & FLAG_BIN_OVERFLOW)) return EUCHIP_OK;
-        // Bin storage overflowed: nothing was written.  Reset and retry larger.
+        const Counters &hc = *ctx->host_counters;
+        if (hc.max_count > ctx->tile_cap) {  // grow the bins for the frames that follow
+            uint32_t c = ctx->tile_cap;
+            while (c < hc.max_count && c < (1u << 24)) c <<= 1;
+            ctx->tile_cap = c;
+        }
+        if (!(hc.flags & FLAG_BIN_OVERFLOW)) return EUCHIP_OK;
+        // The overflow list filled up: nothing was written.  Reset and retry larger.
         HIP_TRY(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(Counters), ctx->stream));
-        const size_t need = (size_t)ctx->host_counters->npairs;
-        if (attempt >= 3 || need >= 0xFFFFFFF0ull) return fail(ctx, EUCHIP_OVER_BUDGET, "bin storage overflow");
-        for (DevArray<uint32_t> *a : {&ctx->pair_prim, &ctx->pair_tile, &ctx->pair_rank, &ctx->bins})
-            if ((s = ensure(ctx, *a, std::min<size_t>(need + need / 4 + 1024, 0xFFFFFFF0ull), "bins"))) return s;
+        HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4, ctx->stream));
+        if (attempt >= 3) return fail(ctx, EUCHIP_OVER_BUDGET, "bin storage overflow");
+        if ((s = ensure(ctx, ctx->ovf, ctx->ovf.cap * 4, "overflow list"))) return s;
     }
 }
 
@@ -346,6 +356,9 @@ euchip_status euchip_ctx_create(int device, euchip_ctx **out) {
     if (hipSetDevice(device) != hipSuccess) return EUCHIP_HIP_ERROR;
     euchip_ctx *ctx = new euchip_ctx();
     ctx->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        ctx->max_groups = (uint32_t)cus * 8u;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void **)&ctx->counters, sizeof(Counters)) != hipSuccess ||
         hipHostMalloc((void **)&ctx->host_counters, sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
@@ -366,13 +379,11 @@ euchip_status euchip_ctx_destroy(euchip_ctx *ctx) {
         (void)hipEventDestroy(p.b);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-    (void)hipFree(ctx->recs.ptr);
+    (void)hipFree(ctx->clip.ptr);
+    (void)hipFree(ctx->vary.ptr);
     (void)hipFree(ctx->tile_count.ptr);
-    (void)hipFree(ctx->tile_off.ptr);
-    (void)hipFree(ctx->pair_prim.ptr);
-    (void)hipFree(ctx->pair_tile.ptr);
-    (void)hipFree(ctx->pair_rank.ptr);
     (void)hipFree(ctx->bins.ptr);
+    (void)hipFree(ctx->ovf.ptr);
     (void)hipFree(ctx->counters);
     if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -390,8 +401,15 @@ euchip_status euchip_synchronize(euchip_ctx *ctx) {
                                 ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->profiling) resolve_pending(ctx);
-    if (ctx->host_counters->flags & FLAG_BIN_OVERFLOW) {
+    const Counters &hc = *ctx->host_counters;
+    if (hc.max_count > ctx->tile_cap) {
+        uint32_t c = ctx->tile_cap;
+        while (c < hc.max_count && c < (1u << 24)) c <<= 1;
+        ctx->tile_cap = c;
+    }
+    if (hc.flags & FLAG_BIN_OVERFLOW) {
         HIP_TRY(ctx, hipMemset(ctx->counters, 0, sizeof(Counters)));
+        HIP_TRY(ctx, hipMemset(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4));
         return fail(ctx, EUCHIP_OVER_BUDGET, "bin storage overflow in an asynchronous frame; it was dropped");
     }
     return EUCHIP_OK;

@@ -13,6 +13,12 @@ namespace euchip {
 
 #define EUCHIP_DEV __device__ __forceinline__
 
+// VertexData array length of a shader (at least 1 so arrays stay legal).
+template <uint32_t SH>
+struct VDN {
+    static constexpr uint32_t N = vd_count(SH) ? vd_count(SH) : 1;
+};
+
 // f32::min / f32::max (the non-NaN operand wins).
 EUCHIP_DEV float rmin(float a, float b) { return fminf(a, b); }
 EUCHIP_DEV float rmax(float a, float b) { return fmaxf(a, b); }

@@ -250,6 +250,11 @@ def rasterize(cfg, verts, blit):
 
 
 def render(cfg, hom, vd, pixel, depth, group_rows=None):
+    with np.errstate(all="ignore"):  # IEEE results (inf, NaN) are the point, not an error
+        return _render(cfg, hom, vd, pixel, depth, group_rows)
+
+
+def _render(cfg, hom, vd, pixel, depth, group_rows=None):
     """pixel: uint32 [H, W] or None; depth: float32 [H, W] or None.  cfg keys:
     shader ('flat'|'color'), depth_test (0..3), depth_write, pixel_write, cull,
     y_up, z_clip ((lo, hi) or None), msaa_level, bands ('par'|'seq')."""

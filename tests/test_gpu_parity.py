@@ -115,3 +115,72 @@ def test_color_vertex_triangle(gpu_ctx):
         d = PipelineDesc.make(Shader.COLOR, Xform.CLIP, cull=cull)
         r = run_pair(gpu_ctx, d, S.uniforms(), mesh, 640, 480, depth=False)
         assert_bit_exact(r, f"triangle.rs cull={cull.name}")
+
+
+# --------------------------------------------------------------------------- #
+# two-pass shadow mapping + texture (the heightfield workload)
+# --------------------------------------------------------------------------- #
+
+def shadow_then_main(gpu_ctx, sc, msaa_level):
+    """Depth-only pass into the light's shadow map, then the textured, lit,
+    shadowed main pass sampling it (examples/teapot.rs:205-222 structure)."""
+    S_ = sc.extra["shadow_size"]
+    sd = PipelineDesc.make(Shader.DEPTH_ONLY, Xform.MVP, pixel_mode=PixelMode.PASS,
+                           depth_mode=DepthMode.LESS_WRITE, cull=CullMode.NONE)
+    r1 = run_pair(gpu_ctx, sd, sc.extra["shadow_uniforms"], sc.mesh, S_, S_, pixel=False)
+    assert_bit_exact(r1, "shadow pass")
+    shadow = r1["o_depth"]
+    md = PipelineDesc.make(Shader.TEX_SHADOW, Xform.MVP, depth_mode=DepthMode.LESS_WRITE,
+                           aa_mode=AaMode.Msaa(msaa_level) if msaa_level else AaMode.NONE)
+    r2 = run_pair(gpu_ctx, md, sc.uniforms, sc.mesh, sc.width, sc.height, texture=sc.texture, shadow=shadow)
+    return r1, r2
+
+
+@pytest.mark.parametrize("msaa", [0, 1, 2])
+def test_heightfield_small(gpu_ctx, msaa):
+    sc = S.heightfield_scene(width=480, height=270, grid=81, tex_size=256, shadow_size=512)
+    r1, r2 = shadow_then_main(gpu_ctx, sc, msaa)
+    assert r1["frags"] > 1000 and r2["frags"] > 1000
+    assert_bit_exact(r2, f"heightfield main msaa={msaa}")
+
+
+def test_heightfield_full(gpu_ctx):
+    """200k-triangle heightfield, 2048^2 shadow map, 1024^2 texture, 1080p, Msaa { level: 1 }."""
+    sc = S.heightfield_scene()
+    r1, r2 = shadow_then_main(gpu_ctx, sc, 1)
+    assert r2["frags"] > 100000
+    assert_bit_exact(r2, "heightfield full")
+
+
+def test_textured_cube(gpu_ctx):
+    """examples/texture_mapping.rs: a textured cube, `e as u8` blend, OpenGL-style projection."""
+    rng = np.random.default_rng(1)
+    tex = rng.integers(0, 2**32, (64, 64), dtype=np.uint64).astype(np.uint32)
+    pos, nrm, idx = S.box(3)
+    uv = (pos[:, :2] * 0.5 + 0.5).astype(np.float32)
+    model = S.f32(S.translation(0, 0, 4.0) @ S.rotation(0.6, 0.8, 0.2)).reshape(1, 16)
+    mesh = S.Mesh(pos=pos, uv=uv, idx=idx, count=int(idx.size), models=model)
+    u = S.uniforms(view_proj=S.f32(S.perspective_fov_lh_zo(1.2, 320, 240, 0.1, 50.0)))
+    for cull in (CullMode.BACK, CullMode.NONE):
+        d = PipelineDesc.make(Shader.TEXTURED, Xform.MVP, depth_mode=DepthMode.LESS_WRITE, cull=cull)
+        r = run_pair(gpu_ctx, d, u, mesh, 320, 240, texture=tex, color_init=180)
+        assert r["frags"] > 1000
+        assert_bit_exact(r, f"textured cull={cull.name}")
+
+
+def test_phong_shadow_point_light(gpu_ctx):
+    """examples/teapot.rs Teapot shader: point light, camera position, shadow lookup."""
+    sc = S.torus_scene(width=400, height=300, nu=48, nv=24, instances=2)
+    lvp = S.f32(S.perspective_fov_lh_zo(0.9, 512, 512, 0.1, 100.0) @ S.look_at_lh((6, 8, -2), (0, 0, 4), (0, 1, 0)))
+    su = S.uniforms(view_proj=lvp)
+    sd = PipelineDesc.make(Shader.DEPTH_ONLY, Xform.MVP, pixel_mode=PixelMode.PASS,
+                           depth_mode=DepthMode.LESS_WRITE, cull=CullMode.NONE)
+    r1 = run_pair(gpu_ctx, sd, su, sc.mesh, 512, 512, pixel=False)
+    assert_bit_exact(r1, "teapot-style shadow pass")
+    u = sc.uniforms.copy()
+    u[S.U_LPOS:S.U_LPOS + 3] = (6, 8, -2)
+    u[S.U_LVP:S.U_LVP + 16] = lvp.reshape(16)
+    d = PipelineDesc.make(Shader.PHONG_SHADOW, Xform.MVP, depth_mode=DepthMode.LESS_WRITE, cull=CullMode.FRONT)
+    r = run_pair(gpu_ctx, d, u, sc.mesh, sc.width, sc.height, shadow=r1["o_depth"])
+    assert r["frags"] > 1000
+    assert_bit_exact(r, "phong shadow")
