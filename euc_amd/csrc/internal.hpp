@@ -48,6 +48,7 @@ enum FrameFlags : uint32_t { FLAG_BIN_OVERFLOW = 1u };
 // Each counter sits on its own 128-byte line.
 struct Counters {
     uint32_t ovf[2][32];
+    uint32_t next[2][8][32];  // per frame parity and XCD: next tile of the dynamic tile loop
     uint32_t flags;      // FrameFlags, sticky until the host clears them
     uint32_t max_count;  // largest per-tile triangle count seen (host resizes bins)
     uint32_t pad[30];

@@ -496,15 +496,22 @@ EUCHIP_DEV void raster_lane(const FrameParams &p, const TriRec &R, uint32_t prim
 // hand it to the pixel-parallel pass.
 constexpr uint32_t kLaneWork = 48;
 
+// Setup fields the pixel-parallel pass needs, kept in LDS.
+struct BigTri {
+    float o[3], dx[3], dy[3], hz[3], vby[6];
+    uint32_t xmin, xmax, ymin, ymax, flags, prim;
+};
+
 struct TileShared {
     uint64_t key[kTileW * kTileH];   // winner key per pixel
     float d0s[kTileW * kTileH];      // depth before the pass (the test's reference)
-    uint32_t big[256];               // triangles handed to the pixel-parallel pass
+    BigTri big[256];                 // triangles handed to the pixel-parallel pass
     uint32_t band[2 * kTileH];       // [by0, by1) of the render_par band of each row
     uint32_t nbig;
+    uint32_t next;                   // tile taken from the dynamic queue
 };
 
-// One bin entry: rasterise it now (lane-parallel) or queue it for the
+// One bin entry: rasterise it now (lane-parallel) or queue its setup for the
 // pixel-parallel pass.
 EUCHIP_DEV void bin_entry(const FrameParams &p, TileShared &sh, uint32_t prim, uint32_t X0t, uint32_t Y0t,
                           uint32_t mode) {
@@ -514,8 +521,16 @@ EUCHIP_DEV void bin_entry(const FrameParams &p, TileShared &sh, uint32_t prim, u
     const uint32_t ry1 = R.ymax < Y0t + kTileH ? R.ymax : Y0t + kTileH;
     const uint32_t xe = R.xmax < X0t + kTileW ? R.xmax : X0t + kTileW;
     const uint32_t cost = (ry1 > ry0 ? ry1 - ry0 : 0u) * (xe > R.xmin ? xe - R.xmin : 0u);
-    if (cost <= kLaneWork) raster_lane(p, R, prim, X0t, Y0t, mode, sh.key, sh.d0s, sh.band);
-    else sh.big[atomicAdd(&sh.nbig, 1u)] = prim;
+    if (cost <= kLaneWork) {
+        raster_lane(p, R, prim, X0t, Y0t, mode, sh.key, sh.d0s, sh.band);
+    } else {
+        BigTri &B = sh.big[atomicAdd(&sh.nbig, 1u)];
+#pragma unroll
+        for (int i = 0; i < 3; i++) B.o[i] = R.o[i], B.dx[i] = R.dx[i], B.dy[i] = R.dy[i], B.hz[i] = R.hz[i];
+#pragma unroll
+        for (int i = 0; i < 6; i++) B.vby[i] = R.vby[i];
+        B.xmin = R.xmin, B.xmax = R.xmax, B.ymin = R.ymin, B.ymax = R.ymax, B.flags = R.flags, B.prim = prim;
+    }
 }
 
 // The queued large triangles, one pixel per thread.
@@ -524,10 +539,14 @@ EUCHIP_DEV void big_pass(const FrameParams &p, TileShared &sh, uint32_t tid, uin
     const uint32_t nb = sh.nbig;
     const uint32_t r = tid >> kTileShiftX;
     for (uint32_t k = 0; k < nb; k++) {
-        const uint32_t prim = __builtin_amdgcn_readfirstlane(sh.big[k]);
+        const BigTri &B = sh.big[k];
+        if (!inside || py < B.ymin || py >= B.ymax) continue;
         TriRec R;
-        tri_setup<true>(p, prim, R);
-        if (!inside || py < R.ymin || py >= R.ymax) continue;
+#pragma unroll
+        for (int i = 0; i < 3; i++) R.o[i] = B.o[i], R.dx[i] = B.dx[i], R.dy[i] = B.dy[i], R.hz[i] = B.hz[i];
+#pragma unroll
+        for (int i = 0; i < 6; i++) R.vby[i] = B.vby[i];
+        R.xmin = B.xmin, R.xmax = B.xmax, R.ymin = B.ymin, R.ymax = B.ymax, R.flags = B.flags;
         uint32_t x0, x1;
         row_span(R, py, sh.band[2 * r], sh.band[2 * r + 1], x0, x1);
         if (px < x0 || px >= x1) continue;
@@ -541,7 +560,7 @@ EUCHIP_DEV void big_pass(const FrameParams &p, TileShared &sh, uint32_t tid, uin
             w2 = w2 + R.dx[2];
         }
         float z;
-        if (frag_z(p, R, w0, w1, w2, z)) update_key(sh.key + tid, mode, p.depth_test, z, D0, prim);
+        if (frag_z(p, R, w0, w1, w2, z)) update_key(sh.key + tid, mode, p.depth_test, z, D0, B.prim);
     }
 }
 
@@ -566,10 +585,15 @@ __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
     const bool uses_depth = p.depth_test != DT_NONE || p.depth_write;
     const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
                         : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
+    // Tiles of this XCD's range: the first by workgroup index, the rest from a
+    // per-XCD counter, so workgroups that drew heavy tiles take fewer of them.
+    const uint32_t xcd = blockIdx.x & 7u;
     uint32_t t0, t1;
-    xcd_range(ntiles, blockIdx.x & 7u, t0, t1);
+    xcd_range(ntiles, xcd, t0, t1);
     const uint32_t per_xcd = gridDim.x >> 3;
-    for (uint32_t tile = t0 + (blockIdx.x >> 3); tile < t1; tile += per_xcd) {
+    uint32_t *next = &p.ctr->next[p.frame & 1u][xcd][0];
+    if (blockIdx.x < 8u && tid == 0u) p.ctr->next[(p.frame + 1u) & 1u][xcd][0] = 0u;  // next frame's queue
+    for (uint32_t tile = t0 + (blockIdx.x >> 3); tile < t1;) {
         const uint32_t ty = tile / p.ntx, tx = tile - ty * p.ntx;
         const uint32_t X0t = tx * kTileW, Y0t = ty * kTileH;
         const uint32_t px = X0t + (tid & (kTileW - 1)), py = Y0t + (tid >> kTileShiftX);
@@ -640,7 +664,10 @@ __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
                 else if (p.clear_depth_en) p.depth[pidx] = p.clear_depth;
             }
         }
+        if (tid == 0u) sh.next = t0 + per_xcd + atomicAdd(next, 1u);
         __syncthreads();  // LDS is reused by the next tile
+        tile = sh.next;
+        __syncthreads();
     }
 }
 
