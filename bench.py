@@ -1,5 +1,8 @@
 """Benchmark of the triangle path on one MI355X.
 
+`--workload torus` (default) measures one 1080p frame per step; `--workload
+icons` measures one batch of 4096 icon renders per step (see bench_icons).
+
 Default workload: four 65,536-triangle tori (256 x 128 segments, seeded model
 matrices) rendered with per-fragment Phong shading (2 directional lights),
 LESS_WRITE depth, back-face culling, into a 1920x1080 RGBA8 colour target and
@@ -33,10 +36,26 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--shader", default="phong", choices=["phong", "gouraud"])
+    ap.add_argument("--workload", default="torus", choices=["torus", "icons"])
+    ap.add_argument("--icons", type=int, default=4096, help="requests per batch of the icons workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
     ap.add_argument("--check", action="store_true", help="also compare one GPU frame with the oracle")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB read + 256 MiB write elementwise kernel three times "
+                         "(the reference point that scales the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def copy_probe():
+    import torch
+    n = 64 << 20  # 64 Mi f32 = 256 MiB each way
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.add(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "dispatches": 3}))
 
 
 def algorithmic_bytes(scene) -> dict:
@@ -72,12 +91,164 @@ def cpu_baseline(scene, desc, seconds: float):
     return {"frame_s": med, "frames": len(times), "threads": threads, "frags": frags, "first": first}
 
 
+ICON_SEED = 0xC0FFEE00
+ICON_CLEAR = 0xFF202020
+
+
+def icon_cpu_baseline(scenes, desc, seconds: float):
+    """The oracle's render over the icon requests, one request per host thread
+    (render_seq: each request is one target), until `seconds` have passed."""
+    import concurrent.futures as cf
+    import threading
+    from oracle import oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    t_end = time.perf_counter() + seconds
+    done = [0]
+    lock = threading.Lock()
+
+    def worker(k):
+        i = k
+        n = 0
+        while time.perf_counter() < t_end:
+            sc = scenes[i % len(scenes)]
+            pix = np.full((sc.height, sc.width), ICON_CLEAR, np.uint32)
+            dep = np.full((sc.height, sc.width), 1.0, np.float32)
+            rc, _ = O.render(desc, sc.uniforms, sc.mesh, pix, dep)
+            assert rc == 0
+            n += 1
+            i += cores
+        with lock:
+            done[0] += n
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(cores) as ex:
+        list(ex.map(worker, range(cores)))
+    el = time.perf_counter() - t0
+    return {"value": done[0] / el, "unit": "icons/s", "cores": cores, "kind": "port",
+            "sample": f"{done[0]} icon requests of this batch (in order, {cores} at a time, one host thread "
+                      f"each) in {el:.1f} s, oracle render_seq"}
+
+
+def bench_icons(args):
+    """Batched icons: `--icons` requests (seeded spheres, tori, boxes of 200-20k
+    log-uniform triangles; square viewports 32..256 px; Phong, LESS_WRITE, back-face
+    culling, render_seq bands, colour and depth cleared in the frame).  One step =
+    one whole batch (a hipGraph replay: one setup and one raster launch for all
+    requests) from launch to completion; the step times give p50/p99 latency.
+    Compared with the plain path: the same requests as one render() each."""
+    from euc_amd import scenes as S
+    from euc_amd.pipeline import BandPolicy, CullMode, DepthMode, PipelineDesc, Shader, Xform
+    from euc_amd.renderer import Context, Request
+
+    n = args.icons
+    desc = PipelineDesc.make(Shader.PHONG, Xform.MVP, depth_mode=DepthMode.LESS_WRITE, cull=CullMode.BACK,
+                             band_policy=BandPolicy.SEQ)
+    scenes = [S.icon_request(ICON_SEED + i) for i in range(n)]
+    ctx = Context(0)
+    meshes = [ctx.mesh(sc.mesh) for sc in scenes]
+    tg = [(ctx.color_target(sc.width, sc.height), ctx.depth_target(sc.width, sc.height)) for sc in scenes]
+    px = sum(sc.width * sc.height for sc in scenes)
+    tris = sum(sc.mesh.n_triangles for sc in scenes)
+
+    # plain path: one render() per request, all enqueued, one wait
+    def plain_pass():
+        for sc, m, (c, d) in zip(scenes, meshes, tg):
+            ctx.render(desc, sc.uniforms, m, c, d, clear_color=ICON_CLEAR, clear_depth=1.0, sync=False)
+        ctx.synchronize()
+
+    plain_pass()
+    plain_pass()
+    sample = list(range(0, n, max(1, n // 64)))
+    ref = {i: (tg[i][0].download(), tg[i][1].download()) for i in sample}
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain_pass()
+    plain_s = (time.perf_counter() - t0) / reps
+    # plain per-request latency: render() alone, waited for
+    lat_plain = []
+    for i in sample:
+        sc, m, (c, d) = scenes[i], meshes[i], tg[i]
+        t = time.perf_counter()
+        ctx.render(desc, sc.uniforms, m, c, d, clear_color=ICON_CLEAR, clear_depth=1.0, sync=True)
+        lat_plain.append(time.perf_counter() - t)
+
+    for c, d in tg:  # the batch must write every target itself
+        c.clear(0x5A5A5A5A)
+        d.clear(0.5)
+    b = ctx.batch().set(desc, [Request(m, c, d, sc.uniforms) for sc, m, (c, d) in zip(scenes, meshes, tg)],
+                        clear_color=ICON_CLEAR, clear_depth=1.0)
+    tile_w, tune_ms = b.autotune(reps=3)
+    for _ in range(args.warmup):
+        b.run(graph=True)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        b.launch(graph=True)
+        b.wait()
+        lat.append(time.perf_counter() - t)
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    parity = all(np.array_equal(tg[i][0].download(), ref[i][0]) and
+                 np.array_equal(tg[i][1].download().view(np.uint32), ref[i][1].view(np.uint32)) for i in sample)
+
+    # per-kernel times of the same batch, eager launches with events on the batch's stream
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    for _ in range(min(args.steps, 20)):
+        b.run(graph=False)
+    ctx.set_profiling(False)
+    stats = ctx.kernel_stats()
+    kernels = {k: {"launches": v[0], "avg_us": v[1] * 1e3 / max(v[0], 1)} for k, v in stats.items()}
+    raster_us = kernels.get("batch_raster", {}).get("avg_us", float("nan"))
+    target_bytes = px * 8
+    achieved = target_bytes / (raster_us * 1e-6) / 1e9 if raster_us == raster_us else None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = icon_cpu_baseline(scenes, desc, args.cpu_seconds)
+    icons_per_s = n / (ms_per_step * 1e-3)
+    lat_ms = np.array(lat) * 1e3
+    line = {
+        "metric": "batched icons/s (4096 Phong icon requests per batch, p50/p99 batch latency)",
+        "value": icons_per_s, "unit": "icons/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n} seeded icon requests (sphere/torus/box, 200-20k tris log-uniform, "
+                               f"32-256 px square RGBA8+f32, phong, LESS_WRITE, cull back, render_seq bands, "
+                               f"clears in frame), one hipGraph replay per batch",
+                   "requests": n, "triangles": tris, "pixels": px, "tile_w": tile_w},
+        "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+        "autotune_ms": tune_ms,
+        "plain": {"icons_per_s": n / plain_s, "ms_per_batch": plain_s * 1e3,
+                  "request_p50_ms": float(np.percentile(np.array(lat_plain) * 1e3, 50)),
+                  "request_p99_ms": float(np.percentile(np.array(lat_plain) * 1e3, 99))},
+        "speedup_vs_plain": icons_per_s / (n / plain_s),
+        "parity_with_plain_path": parity,
+        "kernels_us": kernels,
+        "roofline": {"bound": "hbm", "kernel": "batch_raster", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBPS) if achieved else None, "traffic": None,
+                     "algorithmic_bytes_per_launch": target_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    b.close()
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               f"(the path would shard screen tiles across GPUs, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
+    if args.workload == "icons":
+        bench_icons(args)
+        return
 
     from euc_amd import scenes as S
     from euc_amd.pipeline import DepthMode, PipelineDesc, Shader, Xform

@@ -46,6 +46,32 @@ class MeshDescC(C.Structure):
     ]
 
 
+class RequestC(C.Structure):
+    _fields_ = [
+        ("mesh", C.c_void_p), ("pixel", C.c_void_p), ("depth", C.c_void_p), ("uniforms", C.c_void_p),
+        ("texture", C.c_void_p), ("shadow", C.c_void_p),
+    ]
+
+
+class ServeRequestC(C.Structure):
+    _fields_ = [
+        ("mesh", C.c_void_p), ("texture", C.c_void_p), ("shadow", C.c_void_p), ("uniforms", C.c_void_p),
+        ("width", C.c_uint32), ("height", C.c_uint32),
+    ]
+
+
+class CompletionC(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("status", C.c_int), ("pixel", C.c_void_p), ("depth", C.c_void_p)]
+
+
+class ServerStatsC(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+        ("batches", C.c_uint64), ("queued", C.c_uint32), ("in_flight", C.c_uint32),
+        ("bytes_in_use", C.c_size_t), ("peak_bytes", C.c_size_t), ("budget", C.c_size_t),
+    ]
+
+
 class KernelStatC(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -76,6 +102,26 @@ SIGNATURES = {
                                 C.c_void_p]),
     "euchip_render_async": (C.c_int, [C.c_void_p, C.POINTER(PipelineDescC), C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
+    "euchip_ctx_bytes_in_use": (C.c_size_t, [C.c_void_p]),
+    "euchip_ctx_set_budget": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "euchip_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "euchip_batch_destroy": (C.c_int, [C.c_void_p]),
+    "euchip_batch_set": (C.c_int, [C.c_void_p, C.POINTER(PipelineDescC), C.POINTER(RequestC), C.c_uint32]),
+    "euchip_batch_set_tile_width": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "euchip_batch_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
+    "euchip_batch_launch": (C.c_int, [C.c_void_p, C.c_int]),
+    "euchip_batch_wait": (C.c_int, [C.c_void_p]),
+    "euchip_batch_query": (C.c_int, [C.c_void_p]),
+    "euchip_batch_stream": (C.c_void_p, [C.c_void_p]),
+    "euchip_batch_scratch_bytes": (C.c_size_t, [C.c_void_p]),
+    "euchip_batch_estimate_bytes": (C.c_size_t, [C.POINTER(PipelineDescC), C.POINTER(RequestC), C.c_uint32]),
+    "euchip_server_create": (C.c_int, [C.c_void_p, C.POINTER(PipelineDescC), C.c_size_t, C.c_uint32, C.c_uint32,
+                                       C.POINTER(C.c_void_p)]),
+    "euchip_server_destroy": (C.c_int, [C.c_void_p]),
+    "euchip_server_submit": (C.c_int, [C.c_void_p, C.POINTER(ServeRequestC), C.POINTER(C.c_uint64)]),
+    "euchip_server_poll": (C.c_int, [C.c_void_p, C.POINTER(CompletionC), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "euchip_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "euchip_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStatsC)]),
 }
 
 _lib = None

@@ -6,10 +6,6 @@
 namespace euchip {
 
 constexpr int kUniforms = 64;
-constexpr int kTileW = 32;  // raster tile: 32 x 8 pixels, one pixel per thread of a 256-thread workgroup
-constexpr int kTileH = 8;
-constexpr int kTileShiftX = 5;
-constexpr int kTileShiftY = 3;
 constexpr int kMaxVD = 12;  // VertexData floats of the widest fixed shader
 
 // Uniform block offsets (same layout as include/euchip.h documents).
@@ -38,34 +34,38 @@ struct TriRec {
     uint32_t xmin, xmax;        // bounds_clamped x, clamped to [0, W]
     uint32_t ymin, ymax;        // bounds_clamped y, clamped to [0, H]
     uint32_t flags;             // bit 0: all three vertices pass the z clip
-    uint32_t sid[3];            // shaded-vertex id (instance * n_vertices + vertex), post-reversal
+    uint32_t sid[3];            // shaded-vertex id in clip / vary (request base + instance * n_vertices + vertex), post-reversal
 };
 
 enum FrameFlags : uint32_t { FLAG_BIN_OVERFLOW = 1u };
 
-// Device counters of one context.  ovf[f & 1] counts the overflow entries of
-// frame f; the raster pass of frame f zeroes ovf[(f + 1) & 1] for the next one.
-// Each counter sits on its own 128-byte line.
+// Device counters of one launch stream.  ovf[f & 1] counts the overflow
+// entries of frame f; the raster pass of frame f zeroes ovf[(f + 1) & 1] and
+// next[(f + 1) & 1] for the next one.  Each counter sits on its own 128-byte line.
 struct Counters {
     uint32_t ovf[2][32];
     uint32_t next[2][8][32];  // per frame parity and XCD: next tile of the dynamic tile loop
     uint32_t flags;      // FrameFlags, sticky until the host clears them
-    uint32_t max_count;  // largest per-tile triangle count seen (host resizes bins)
+    uint32_t max_count;  // largest per-tile triangle count seen by a one-request frame
     uint32_t pad[30];
 };
 
-struct FrameParams {
+// One render request: a target pair, a mesh, a uniform block, and where the
+// request's vertices, tiles and bins sit in the launch's scratch memory.
+struct View {
     // target (tgt_size) and the two targets; null = Empty
     uint32_t W, H;
+    uint32_t G;            // render_par band height (rows), H for render_seq
+    uint32_t ntx, nty;     // tiles of the target
+    uint32_t tile_base;    // first tile of this request in the launch's tile space
+    uint32_t sv_base;      // first shaded vertex of this request in clip / vary
+    uint32_t tile_cap;     // bin slots per tile of this request
+    uint64_t bin_base;     // first bin slot of this request
     uint32_t *pixel;
     float *depth;
-    // pipeline
-    uint32_t shader, xform, pixel_write, depth_test, depth_write, y_up, z_clip;
-    float z_min, z_max;
-    uint32_t cull, msaa_level, band_par, G;
     uint32_t clear_color_en, clear_color, clear_depth_en;
     float clear_depth;
-    float ndc_x, ndc_y;   // 2/size_x and -2/size_y of to_ndc (triangles.rs:44-48)
+    float ndc_x, ndc_y;    // 2/size_x and -2/size_y of to_ndc (triangles.rs:44-48)
     // mesh (device)
     const float *pos;
     const float *nrm;
@@ -80,23 +80,39 @@ struct FrameParams {
     uint32_t tex_w, tex_h;
     const float *shadow;
     uint32_t sh_w, sh_h;
-    // per-frame scratch (device)
-    float4 *clip;          // clip position per shaded vertex (instance * n_vertices + vertex)
-    float *vary;           // VertexData per shaded vertex (instance * n_vertices + vertex)
-    Counters *ctr;
-    uint32_t *tile_count;  // per tile, zero between frames
-    uint32_t *bins;        // tile * tile_cap + rank -> primitive
-    uint32_t tile_cap;     // bin slots per tile
-    uint2 *ovf;            // (tile, primitive) of ranks >= tile_cap
-    uint32_t ovf_cap;
-    uint32_t frame;        // frame parity selects the overflow counter
-    uint32_t ntx, nty;
     float u[kUniforms];
 };
 
-// Kernel launchers (kernels.hip).  All enqueue on `s`.
-hipError_t launch_setup(const FrameParams &p, hipStream_t s);
-hipError_t launch_raster(const FrameParams &p, hipStream_t s, uint32_t max_groups);
+struct FrameParams {
+    // pipeline state shared by every request of the launch
+    uint32_t shader, xform, pixel_write, depth_test, depth_write, y_up, z_clip;
+    float z_min, z_max;
+    uint32_t cull, msaa_level, band_par;
+    // per-launch scratch (device)
+    float4 *clip;          // clip position per shaded vertex
+    float *vary;           // VertexData per shaded vertex
+    Counters *ctr;
+    uint32_t *tile_count;  // per tile of the launch, zero between frames
+    uint32_t *bins;        // bin_base + local tile * tile_cap + rank -> primitive
+    uint32_t *req_max;     // per request: largest tile count above its tile_cap (host grows the bins)
+    uint2 *ovf;            // (tile, primitive) of ranks >= tile_cap
+    uint32_t ovf_cap;
+    uint32_t frame;        // frame parity selects the overflow counter and tile queue
+    uint32_t ntiles;       // tiles of all requests
+    uint32_t tile_sx;      // tiles are (1 << tile_sx) x (256 >> tile_sx) pixels, tile_sx in 3..5
+    // requests: one in `v0` (kernel argument), or `n_views` in device memory
+    uint32_t n_views;
+    const View *views;
+    const uint2 *blk;          // batched setup: workgroup -> (request, workgroup within the request)
+    const uint32_t *tile_req;  // batched raster: tile -> request
+    View v0;
+};
+
+// Kernel launchers (kernels.hip).  All enqueue on `s`.  `batched` selects the
+// kernels that read their requests from p.views (setup then runs `blocks`
+// workgroups as p.blk maps them); one-request launches read p.v0.
+hipError_t launch_setup(const FrameParams &p, hipStream_t s, bool batched, uint32_t blocks);
+hipError_t launch_raster(const FrameParams &p, hipStream_t s, bool batched, uint32_t max_groups);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s);
 
 }  // namespace euchip

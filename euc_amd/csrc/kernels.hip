@@ -1,24 +1,26 @@
 // kernels.hip -- gfx950 kernels of the triangle path.
 //
-// Frame = setup -> raster (two launches):
-//   k_setup   one lane per triangle: vertex positions (the fixed vertex shader),
-//             euc's triangle setup (triangles.rs:55-171) in IEEE f32, culling,
-//             a 128-byte record at the primitive's index, and the primitive
-//             appended to the bin of every 32x8 tile its bounding box covers
-//             (slot tile * cap + rank, rank from a per-tile atomic counter;
-//             ranks past the capacity go to a shared overflow list).  The same
-//             launch runs the vertex shader's VertexData once per shaded vertex.
-//   k_raster  256-thread workgroups that loop over tiles (XCD-contiguous), one
-//             pixel per thread.  Each pixel keeps a 64-bit key in LDS that
-//             encodes the winning fragment; the depth rule becomes an
-//             order-independent reduction (atomic min/max of the key) because
-//             every fixed shader's blend replaces the old pixel: the winner is a
-//             function of the set of fragments, with primitive order as the
-//             tie-break (see init_key).  Triangles that touch few pixels of the
-//             tile are rasterised one per lane, walking their rows exactly as
-//             triangles.rs:219-303 does; larger ones are tested pixel-parallel.
-//             The winner is then shaded once and colour and depth are written
-//             once, in full 128-byte rows.
+// Frame = setup -> raster (two launches), for one request (the view in the
+// kernel argument) or for a batch of requests (views in device memory, each
+// with its own target, mesh, uniforms, tiles and bins):
+//   k_setup   one lane per shaded vertex (clip position and VertexData of the
+//             fixed vertex shader) and one lane per triangle: euc's triangle
+//             setup (triangles.rs:55-171) in IEEE f32, culling, and binning of
+//             the triangle into every tile its bounding box covers.  Pairs are
+//             grouped by tile in an LDS hash so each distinct tile costs one
+//             global atomic per workgroup; ranks past a tile's bin capacity go
+//             to a shared overflow list.
+//   k_raster  256-thread workgroups that take tiles (256 pixels, one per
+//             thread; 32x8, 16x16 or 8x32) from per-XCD queues.  Each pixel
+//             keeps a 64-bit key in LDS that encodes the winning fragment; the
+//             depth rule becomes an order-independent reduction (atomic min/max
+//             of the key) because every fixed shader's blend replaces the old
+//             pixel: the winner is a function of the set of fragments, with
+//             primitive order as the tie-break (see init_key).  Triangles that
+//             touch few pixels of the tile are rasterised one per lane, walking
+//             their rows exactly as triangles.rs:219-303 does; larger ones are
+//             tested pixel-parallel.  The winner is then shaded once and colour
+//             and depth are written once per pixel.
 #include "internal.hpp"
 #include "shade.hpp"
 
@@ -79,29 +81,29 @@ EUCHIP_DEV void for_each_tile(bool act, uint32_t tx0, uint32_t ty0, uint32_t nx,
 }
 
 // ---------------------------------------------------------------------------
-// triangle setup (triangles.rs:55-171) of primitive t, whole-target band clamp.
-// CLIPBUF: read the vertices' clip positions from p.clip (written by k_setup)
-// instead of running the vertex shader.  Returns false when the triangle is
-// culled or its bounding box is empty.
+// triangle setup (triangles.rs:55-171) of primitive t of request v, whole-target
+// band clamp.  CLIPBUF: read the vertices' clip positions from p.clip (written
+// by k_setup) instead of running the vertex shader.  Returns false when the
+// triangle is culled or its bounding box is empty.
 // ---------------------------------------------------------------------------
 template <bool CLIPBUF>
-EUCHIP_DEV bool tri_setup(const FrameParams &p, uint32_t t, TriRec &R) {
+EUCHIP_DEV bool tri_setup(const FrameParams &p, const View &v, uint32_t t, TriRec &R) {
     float hom[3][4], euc[3][3];
     uint32_t sid[3];
     const float flip1 = p.y_up ? -1.0f : 1.0f;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const uint32_t e = 3u * t + (uint32_t)i;
-        const uint32_t inst = e / p.count;
-        const uint32_t k = e - inst * p.count;
-        const uint32_t vi = p.idx ? p.idx[k] : k;
-        sid[i] = inst * p.n_vertices + vi;
+        const uint32_t inst = e / v.count;
+        const uint32_t k = e - inst * v.count;
+        const uint32_t vi = v.idx ? v.idx[k] : k;
+        sid[i] = v.sv_base + inst * v.n_vertices + vi;
         float c[4];
         if (CLIPBUF) {
             const float4 q = p.clip[sid[i]];
             c[0] = q.x, c[1] = q.y, c[2] = q.z, c[3] = q.w;
         } else {
-            vertex_clip(p, inst, vi, c);
+            vertex_clip(p, v, inst, vi, c);
         }
         hom[i][0] = c[0] * 1.0f;
         hom[i][1] = c[1] * flip1;
@@ -137,8 +139,8 @@ EUCHIP_DEV bool tri_setup(const FrameParams &p, uint32_t t, TriRec &R) {
         sid[0] = sid[2];
         sid[2] = ts;
     }
-    const float sx = (float)p.W, sy = (float)p.H;
-    const float T[3][3] = {{p.ndc_x, 0.0f, -1.0f}, {0.0f, p.ndc_y, 1.0f}, {0.0f, 0.0f, 1.0f}};
+    const float sx = (float)v.W, sy = (float)v.H;
+    const float T[3][3] = {{v.ndc_x, 0.0f, -1.0f}, {0.0f, v.ndc_y, 1.0f}, {0.0f, 0.0f, 1.0f}};
     // coords_to_weights (triangles.rs:86-102)
     const float c3[3] = {hom[2][0], hom[2][1], hom[2][3]};
     const float ca[3] = {hom[0][0] - c3[0], hom[0][1] - c3[1], hom[0][3] - c3[2]};
@@ -166,10 +168,10 @@ EUCHIP_DEV bool tri_setup(const FrameParams &p, uint32_t t, TriRec &R) {
         vs[i][0] = sx * (euc[i][0] * 0.5f + 0.5f);
         vs[i][1] = sy * (euc[i][1] * -0.5f + 0.5f);
     }
-    R.xmin = sat_clamp(rmin(rmin(vs[0][0], vs[1][0]), vs[2][0]) + 0.0f, 0u, p.W);
-    R.ymin = sat_clamp(rmin(rmin(vs[0][1], vs[1][1]), vs[2][1]) + 0.0f, 0u, p.H);
-    R.xmax = sat_clamp(rmax(rmax(vs[0][0], vs[1][0]), vs[2][0]) + 1.0f, 0u, p.W);
-    R.ymax = sat_clamp(rmax(rmax(vs[0][1], vs[1][1]), vs[2][1]) + 1.0f, 0u, p.H);
+    R.xmin = sat_clamp(rmin(rmin(vs[0][0], vs[1][0]), vs[2][0]) + 0.0f, 0u, v.W);
+    R.ymin = sat_clamp(rmin(rmin(vs[0][1], vs[1][1]), vs[2][1]) + 0.0f, 0u, v.H);
+    R.xmax = sat_clamp(rmax(rmax(vs[0][0], vs[1][0]), vs[2][0]) + 1.0f, 0u, v.W);
+    R.ymax = sat_clamp(rmax(rmax(vs[0][1], vs[1][1]), vs[2][1]) + 1.0f, 0u, v.H);
     // w_hom at (0,0) and the per-pixel steps (triangles.rs:142-145)
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -206,17 +208,29 @@ EUCHIP_DEV bool tri_setup(const FrameParams &p, uint32_t t, TriRec &R) {
     return R.xmin < R.xmax && R.ymin < R.ymax;
 }
 
-// Bin append of one (tile, primitive) pair by global atomics (the fallback of
-// the LDS-grouped path).
-EUCHIP_DEV void bin_direct(const FrameParams &p, uint32_t tile, uint32_t prim) {
-    const uint32_t rank = atomicAdd(p.tile_count + tile, 1u);
-    if (rank < p.tile_cap) {
-        p.bins[(size_t)tile * p.tile_cap + rank] = prim;
+// Where rank `rank` of tile `tile` (launch tile space) of request v goes: its
+// bin slot, or the overflow list past the tile's capacity.
+EUCHIP_DEV void bin_store(const FrameParams &p, const View &v, uint32_t tile, uint32_t rank, uint32_t prim) {
+    if (rank < v.tile_cap) {
+        p.bins[v.bin_base + (size_t)(tile - v.tile_base) * v.tile_cap + rank] = prim;
     } else {
         const uint32_t o = atomicAdd(&p.ctr->ovf[p.frame & 1u][0], 1u);
         if (o < p.ovf_cap) p.ovf[o] = make_uint2(tile, prim);
         else atomicOr(&p.ctr->flags, FLAG_BIN_OVERFLOW);
     }
+}
+
+// Bin append of one pair by a global atomic (the fallback of the LDS-grouped path).
+EUCHIP_DEV void bin_direct(const FrameParams &p, const View &v, uint32_t tile, uint32_t prim) {
+    bin_store(p, v, tile, atomicAdd(p.tile_count + tile, 1u), prim);
+}
+
+// The request a workgroup or tile works on: the kernel argument's, or one of
+// the batch's in device memory.
+template <bool BATCH>
+EUCHIP_DEV const View &view_of(const FrameParams &p, uint32_t r) {
+    if constexpr (BATCH) return p.views[r];
+    else return p.v0;
 }
 
 constexpr uint32_t kHash = 1024;   // LDS hash slots (distinct tiles) per setup workgroup
@@ -229,42 +243,52 @@ constexpr uint32_t kPairs = 2048;  // (tile, primitive) pairs grouped in LDS per
 // distinct tile costs one global atomic per workgroup; ranks inside the
 // group come from LDS.
 // ---------------------------------------------------------------------------
-template <uint32_t SH>
+template <uint32_t SH, bool BATCH>
 __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
     __shared__ uint32_t hkey[kHash], hcnt[kHash], hbase[kHash];
     __shared__ uint32_t pl_prim[kPairs], pl_slot[kPairs], pl_local[kPairs];
     __shared__ uint32_t npl;
     const uint32_t tid = threadIdx.x;
-    const uint32_t t = blockIdx.x * 256u + tid;
+    uint32_t r = 0, j = blockIdx.x;
+    if constexpr (BATCH) {
+        const uint2 bj = p.blk[blockIdx.x];
+        r = bj.x, j = bj.y;
+    }
+    const View &v = view_of<BATCH>(p, r);
+    const uint32_t t = j * 256u + tid;
     for (uint32_t i = tid; i < kHash; i += 256u) hkey[i] = 0u, hcnt[i] = 0u;
     if (tid == 0u) npl = 0u;
-    if (t < p.n_instances * p.n_vertices) {
-        const uint32_t inst = t / p.n_vertices;
-        const uint32_t vi = t - inst * p.n_vertices;
+    if (t < v.n_instances * v.n_vertices) {
+        const uint32_t inst = t / v.n_vertices;
+        const uint32_t vi = t - inst * v.n_vertices;
+        const size_t sv = (size_t)v.sv_base + t;
         float c[4];
-        vertex_clip(p, inst, vi, c);
-        p.clip[t] = make_float4(c[0], c[1], c[2], c[3]);
+        vertex_clip(p, v, inst, vi, c);
+        p.clip[sv] = make_float4(c[0], c[1], c[2], c[3]);
         if constexpr (vd_count(SH) > 0) {
             float vd[VDN<SH>::N];
-            vertex_data<SH>(p, inst, vi, vd);
+            vertex_data<SH>(p, v, inst, vi, vd);
 #pragma unroll
-            for (uint32_t q = 0; q < vd_count(SH); q++) p.vary[(size_t)t * vd_count(SH) + q] = vd[q];
+            for (uint32_t q = 0; q < vd_count(SH); q++) p.vary[sv * vd_count(SH) + q] = vd[q];
         }
     }
+    const uint32_t sx = p.tile_sx, sy = 8u - p.tile_sx;
     bool keep = false;
     uint32_t tx0 = 0, ty0 = 0, nx = 0, ny = 0;
-    if (t < p.n_tris) {
+    if (t < v.n_tris) {
         TriRec R;
-        keep = tri_setup<false>(p, t, R);
+        keep = tri_setup<false>(p, v, t, R);
         if (keep) {
-            tx0 = R.xmin >> kTileShiftX;
-            ty0 = R.ymin >> kTileShiftY;
-            nx = ((R.xmax - 1u) >> kTileShiftX) - tx0 + 1u;
-            ny = ((R.ymax - 1u) >> kTileShiftY) - ty0 + 1u;
+            tx0 = R.xmin >> sx;
+            ty0 = R.ymin >> sy;
+            nx = ((R.xmax - 1u) >> sx) - tx0 + 1u;
+            ny = ((R.ymax - 1u) >> sy) - ty0 + 1u;
         }
     }
     __syncthreads();
-    for_each_tile(keep, tx0, ty0, nx, ny, p.ntx, t, 0u, [&](uint32_t tile, uint32_t, uint32_t prim, uint32_t) {
+    const uint32_t tbase = v.tile_base;
+    for_each_tile(keep, tx0, ty0, nx, ny, v.ntx, t, 0u, [&](uint32_t lt, uint32_t, uint32_t prim, uint32_t) {
+        const uint32_t tile = tbase + lt;
         uint32_t h = (tile * 2654435761u) >> 22;  // 10-bit multiplicative hash
         for (uint32_t probe = 0; probe < 16u; probe++) {
             const uint32_t old = atomicCAS(&hkey[h], 0u, tile + 1u);
@@ -280,7 +304,7 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
             }
             h = (h + 1u) & (kHash - 1u);
         }
-        bin_direct(p, tile, prim);  // table or pair list full
+        bin_direct(p, v, tile, prim);  // table or pair list full
     });
     __syncthreads();
     for (uint32_t i = tid; i < kHash; i += 256u)
@@ -288,14 +312,8 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
     __syncthreads();
     const uint32_t np = npl < kPairs ? npl : kPairs;
     for (uint32_t k = tid; k < np; k += 256u) {
-        const uint32_t h = pl_slot[k], tile = hkey[h] - 1u, rank = hbase[h] + pl_local[k], prim = pl_prim[k];
-        if (rank < p.tile_cap) {
-            p.bins[(size_t)tile * p.tile_cap + rank] = prim;
-        } else {
-            const uint32_t o = atomicAdd(&p.ctr->ovf[p.frame & 1u][0], 1u);
-            if (o < p.ovf_cap) p.ovf[o] = make_uint2(tile, prim);
-            else atomicOr(&p.ctr->flags, FLAG_BIN_OVERFLOW);
-        }
+        const uint32_t h = pl_slot[k];
+        bin_store(p, v, hkey[h] - 1u, hbase[h] + pl_local[k], pl_prim[k]);
     }
 }
 
@@ -376,13 +394,13 @@ EUCHIP_DEV void row_span(const TriRec &R, uint32_t y, uint32_t by0, uint32_t by1
     x1 = (ce >= fxmin && ce < fxmax) ? (uint32_t)ce : R.xmax;
 }
 
-EUCHIP_DEV void band_of(const FrameParams &p, uint32_t y, uint32_t &by0, uint32_t &by1) {
+EUCHIP_DEV void band_of(const FrameParams &p, const View &v, uint32_t y, uint32_t &by0, uint32_t &by1) {
     if (p.band_par) {
-        by0 = (y / p.G) * p.G;
-        by1 = by0 + p.G < p.H ? by0 + p.G : p.H;
+        by0 = (y / v.G) * v.G;
+        by1 = by0 + v.G < v.H ? by0 + v.G : v.H;
     } else {
         by0 = 0;
-        by1 = p.H;
+        by1 = v.H;
     }
 }
 
@@ -404,8 +422,8 @@ EUCHIP_DEV void v_data(const TriRec &R, const float (*V)[VDN<SH>::N], float X, f
 
 // Deferred shading of one pixel's winning fragment (pipeline.rs:539-576).
 template <uint32_t SH>
-EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const TriRec &R, uint32_t px, uint32_t py, uint32_t by0,
-                                uint32_t old) {
+EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const View &v, const TriRec &R, uint32_t px, uint32_t py,
+                                uint32_t by0, uint32_t old) {
     constexpr uint32_t NV = VDN<SH>::N;
     float V[3][NV];
 #pragma unroll
@@ -418,7 +436,7 @@ EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const TriRec &R, uint32_t 
     const uint32_t L = p.msaa_level;
     if (L == 0u) {
         v_data<SH>(R, V, (float)px, (float)py, vd);
-        fragment_shader<SH>(p, vd, frag);
+        fragment_shader<SH>(v, vd, frag);
     } else {
         // coarse shading on the band-relative 2^L grid with bilinear reconstruction
         // (pipeline.rs:541-570); tgt_min = (0, by0)
@@ -432,7 +450,7 @@ EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const TriRec &R, uint32_t 
         for (int q = 0; q < 4; q++) {
             const uint32_t ix = cx + (uint32_t)(q & 1), iy = cy + (uint32_t)(q >> 1);
             v_data<SH>(R, V, (float)(ix << L), (float)(by0 + (iy << L)), vd);
-            fragment_shader<SH>(p, vd, t[q]);
+            fragment_shader<SH>(v, vd, t[q]);
         }
         if constexpr (SH == FLAT) {
             frag[0] = t[0][0];
@@ -456,18 +474,23 @@ EUCHIP_DEV bool frag_z(const FrameParams &p, const TriRec &R, float w0, float w1
     return (R.flags & 1u) || passes_z_clip(p, z);
 }
 
+// Tile geometry: TW x TH pixels at (X0, Y0); TW = 1 << sx, TH = 256 >> sx.
+struct TileGeom {
+    uint32_t X0, Y0, sx, TW, TH;
+};
+
 // A lane rasterises one triangle over the rows it has in this tile, walking
 // each row from the row start exactly as the reference does.
-EUCHIP_DEV void raster_lane(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t X0t, uint32_t Y0t,
-                            uint32_t mode, uint64_t *key, const float *d0s, const uint32_t *band) {
-    const uint32_t ry0 = R.ymin > Y0t ? R.ymin : Y0t;
-    const uint32_t ry1 = R.ymax < Y0t + kTileH ? R.ymax : Y0t + kTileH;
+EUCHIP_DEV void raster_lane(const FrameParams &p, const TriRec &R, uint32_t prim, const TileGeom &g, uint32_t mode,
+                            uint64_t *key, const float *d0s, const uint32_t *band) {
+    const uint32_t ry0 = R.ymin > g.Y0 ? R.ymin : g.Y0;
+    const uint32_t ry1 = R.ymax < g.Y0 + g.TH ? R.ymax : g.Y0 + g.TH;
     for (uint32_t y = ry0; y < ry1; y++) {
         uint32_t x0, x1;
-        const uint32_t r = y - Y0t;
+        const uint32_t r = y - g.Y0;
         row_span(R, y, band[2 * r], band[2 * r + 1], x0, x1);
-        const uint32_t xs = x0 > X0t ? x0 : X0t;
-        const uint32_t xe = x1 < X0t + kTileW ? x1 : X0t + kTileW;
+        const uint32_t xs = x0 > g.X0 ? x0 : g.X0;
+        const uint32_t xe = x1 < g.X0 + g.TW ? x1 : g.X0 + g.TW;
         if (xs >= xe) continue;
         const float Y = (float)y, X0 = (float)x0;
         float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
@@ -478,11 +501,11 @@ EUCHIP_DEV void raster_lane(const FrameParams &p, const TriRec &R, uint32_t prim
             w1 = w1 + R.dx[1];
             w2 = w2 + R.dx[2];
         }
-        const uint32_t row = (y - Y0t) * kTileW;
+        const uint32_t row = r << g.sx;
         for (uint32_t x = xs; x < xe; x++) {
             float z;
             if (frag_z(p, R, w0, w1, w2, z)) {
-                const uint32_t pi = row + (x - X0t);
+                const uint32_t pi = row + (x - g.X0);
                 update_key(key + pi, mode, p.depth_test, z, d0s[pi], prim);
             }
             w0 = w0 + R.dx[0];
@@ -495,6 +518,7 @@ EUCHIP_DEV void raster_lane(const FrameParams &p, const TriRec &R, uint32_t prim
 // Lanes whose triangle would cost more than this many row steps in the tile
 // hand it to the pixel-parallel pass.
 constexpr uint32_t kLaneWork = 48;
+constexpr uint32_t kMaxTileH = 32;
 
 // Setup fields the pixel-parallel pass needs, kept in LDS.
 struct BigTri {
@@ -503,26 +527,26 @@ struct BigTri {
 };
 
 struct TileShared {
-    uint64_t key[kTileW * kTileH];   // winner key per pixel
-    float d0s[kTileW * kTileH];      // depth before the pass (the test's reference)
-    BigTri big[256];                 // triangles handed to the pixel-parallel pass
-    uint32_t band[2 * kTileH];       // [by0, by1) of the render_par band of each row
+    uint64_t key[256];              // winner key per pixel
+    float d0s[256];                 // depth before the pass (the test's reference)
+    BigTri big[256];                // triangles handed to the pixel-parallel pass
+    uint32_t band[2 * kMaxTileH];   // [by0, by1) of the render_par band of each row
     uint32_t nbig;
-    uint32_t next;                   // tile taken from the dynamic queue
+    uint32_t next;                  // tile taken from the dynamic queue
 };
 
 // One bin entry: rasterise it now (lane-parallel) or queue its setup for the
 // pixel-parallel pass.
-EUCHIP_DEV void bin_entry(const FrameParams &p, TileShared &sh, uint32_t prim, uint32_t X0t, uint32_t Y0t,
+EUCHIP_DEV void bin_entry(const FrameParams &p, const View &v, TileShared &sh, uint32_t prim, const TileGeom &g,
                           uint32_t mode) {
     TriRec R;
-    tri_setup<true>(p, prim, R);
-    const uint32_t ry0 = R.ymin > Y0t ? R.ymin : Y0t;
-    const uint32_t ry1 = R.ymax < Y0t + kTileH ? R.ymax : Y0t + kTileH;
-    const uint32_t xe = R.xmax < X0t + kTileW ? R.xmax : X0t + kTileW;
+    tri_setup<true>(p, v, prim, R);
+    const uint32_t ry0 = R.ymin > g.Y0 ? R.ymin : g.Y0;
+    const uint32_t ry1 = R.ymax < g.Y0 + g.TH ? R.ymax : g.Y0 + g.TH;
+    const uint32_t xe = R.xmax < g.X0 + g.TW ? R.xmax : g.X0 + g.TW;
     const uint32_t cost = (ry1 > ry0 ? ry1 - ry0 : 0u) * (xe > R.xmin ? xe - R.xmin : 0u);
     if (cost <= kLaneWork) {
-        raster_lane(p, R, prim, X0t, Y0t, mode, sh.key, sh.d0s, sh.band);
+        raster_lane(p, R, prim, g, mode, sh.key, sh.d0s, sh.band);
     } else {
         BigTri &B = sh.big[atomicAdd(&sh.nbig, 1u)];
 #pragma unroll
@@ -535,9 +559,9 @@ EUCHIP_DEV void bin_entry(const FrameParams &p, TileShared &sh, uint32_t prim, u
 
 // The queued large triangles, one pixel per thread.
 EUCHIP_DEV void big_pass(const FrameParams &p, TileShared &sh, uint32_t tid, uint32_t px, uint32_t py, bool inside,
-                         uint32_t mode, float D0) {
+                         uint32_t sx, uint32_t mode, float D0) {
     const uint32_t nb = sh.nbig;
-    const uint32_t r = tid >> kTileShiftX;
+    const uint32_t r = tid >> sx;
     for (uint32_t k = 0; k < nb; k++) {
         const BigTri &B = sh.big[k];
         if (!inside || py < B.ymin || py >= B.ymax) continue;
@@ -574,17 +598,18 @@ EUCHIP_DEV void xcd_range(uint32_t ntiles, uint32_t xcd, uint32_t &beg, uint32_t
     end = beg + q + (xcd < r ? 1u : 0u);
 }
 
-template <uint32_t SH>
+template <uint32_t SH, bool BATCH>
 __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
     __shared__ TileShared sh;
     if (p.ctr->flags & FLAG_BIN_OVERFLOW) return;
     const uint32_t tid = threadIdx.x;
-    const uint32_t ntiles = p.ntx * p.nty;
+    const uint32_t ntiles = p.ntiles;
     if (blockIdx.x == 0u && tid == 0u) p.ctr->ovf[(p.frame + 1u) & 1u][0] = 0u;  // next frame's list
     const uint32_t novf = p.ctr->ovf[p.frame & 1u][0] < p.ovf_cap ? p.ctr->ovf[p.frame & 1u][0] : p.ovf_cap;
     const bool uses_depth = p.depth_test != DT_NONE || p.depth_write;
     const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
                         : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
+    const uint32_t sx = p.tile_sx, sy = 8u - sx;
     // Tiles of this XCD's range: the first by workgroup index, the rest from a
     // per-XCD counter, so workgroups that drew heavy tiles take fewer of them.
     const uint32_t xcd = blockIdx.x & 7u;
@@ -594,42 +619,49 @@ __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
     uint32_t *next = &p.ctr->next[p.frame & 1u][xcd][0];
     if (blockIdx.x < 8u && tid == 0u) p.ctr->next[(p.frame + 1u) & 1u][xcd][0] = 0u;  // next frame's queue
     for (uint32_t tile = t0 + (blockIdx.x >> 3); tile < t1;) {
-        const uint32_t ty = tile / p.ntx, tx = tile - ty * p.ntx;
-        const uint32_t X0t = tx * kTileW, Y0t = ty * kTileH;
-        const uint32_t px = X0t + (tid & (kTileW - 1)), py = Y0t + (tid >> kTileShiftX);
-        const bool inside = px < p.W && py < p.H;
-        const size_t pidx = (size_t)py * p.W + px;
+        uint32_t r = 0;
+        if constexpr (BATCH) r = __builtin_amdgcn_readfirstlane(p.tile_req[tile]);
+        const View &v = view_of<BATCH>(p, r);
+        const uint32_t lt = tile - v.tile_base;
+        const uint32_t ty = lt / v.ntx, tx = lt - ty * v.ntx;
+        TileGeom g;
+        g.sx = sx, g.TW = 1u << sx, g.TH = 256u >> sx;
+        g.X0 = tx << sx, g.Y0 = ty << sy;
+        const uint32_t px = g.X0 + (tid & (g.TW - 1u)), py = g.Y0 + (tid >> sx);
+        const bool inside = px < v.W && py < v.H;
+        const size_t pidx = (size_t)py * v.W + px;
         float D0 = 0.0f;
-        if (inside && uses_depth) D0 = p.clear_depth_en ? p.clear_depth : p.depth[pidx];
+        if (inside && uses_depth) D0 = v.clear_depth_en ? v.clear_depth : v.depth[pidx];
         sh.key[tid] = init_key(mode, D0);
         sh.d0s[tid] = D0;
         if (tid == 0u) sh.nbig = 0u;
-        if (tid < kTileH) {
+        if (tid < g.TH) {
             uint32_t by0, by1;
-            band_of(p, Y0t + tid, by0, by1);
+            band_of(p, v, g.Y0 + tid, by0, by1);
             sh.band[2 * tid] = by0;
             sh.band[2 * tid + 1] = by1;
         }
+        const uint32_t cap = v.tile_cap;
         const uint32_t cnt = p.tile_count[tile];
         __syncthreads();
         if (tid == 0u) {
             p.tile_count[tile] = 0u;  // counter of the next frame
-            if (cnt > p.tile_cap) atomicMax(&p.ctr->max_count, cnt);
+            if (cnt > cap) atomicMax(p.req_max + r, cnt);
         }
         // this tile's bin, then (only when it overflowed) its entries of the overflow list
-        const uint32_t nb = cnt < p.tile_cap ? cnt : p.tile_cap;
-        const uint32_t *bin = p.bins + (size_t)tile * p.tile_cap;
-        const uint32_t nscan = nb + (cnt > p.tile_cap ? novf : 0u);
+        const uint32_t nb = cnt < cap ? cnt : cap;
+        const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
+        const uint32_t nscan = nb + (cnt > cap ? novf : 0u);
         for (uint32_t b = 0; b < nscan; b += 256u) {
             const uint32_t i = b + tid;
             if (i < nb) {
-                bin_entry(p, sh, bin[i], X0t, Y0t, mode);
+                bin_entry(p, v, sh, bin[i], g, mode);
             } else if (i < nscan) {
                 const uint2 e = p.ovf[i - nb];
-                if (e.x == tile) bin_entry(p, sh, e.y, X0t, Y0t, mode);
+                if (e.x == tile) bin_entry(p, v, sh, e.y, g, mode);
             }
             __syncthreads();
-            big_pass(p, sh, tid, px, py, inside, mode, D0);
+            big_pass(p, sh, tid, px, py, inside, sx, mode, D0);
             __syncthreads();
             if (tid == 0u) sh.nbig = 0u;
             __syncthreads();
@@ -649,24 +681,24 @@ __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
                 const uint32_t o = (uint32_t)(kk >> 32);
                 z = zunord(mode == 1u ? o : ~o, (uint32_t)kk & 1u);
             }
-            if (p.pixel) {
+            if (v.pixel) {
                 if (p.pixel_write && has && SH != DEPTH_ONLY) {
                     TriRec R;
-                    tri_setup<true>(p, prim, R);
-                    p.pixel[pidx] = shade_pixel<SH>(p, R, px, py, sh.band[2 * (tid >> kTileShiftX)],
-                                                    p.clear_color_en ? p.clear_color : 0u);
-                } else if (p.clear_color_en) {
-                    p.pixel[pidx] = p.clear_color;
+                    tri_setup<true>(p, v, prim, R);
+                    v.pixel[pidx] = shade_pixel<SH>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
+                                                    v.clear_color_en ? v.clear_color : 0u);
+                } else if (v.clear_color_en) {
+                    v.pixel[pidx] = v.clear_color;
                 }
             }
-            if (p.depth) {
-                if (has && p.depth_write) p.depth[pidx] = z;
-                else if (p.clear_depth_en) p.depth[pidx] = p.clear_depth;
+            if (v.depth) {
+                if (has && p.depth_write) v.depth[pidx] = z;
+                else if (v.clear_depth_en) v.depth[pidx] = v.clear_depth;
             }
         }
         if (tid == 0u) sh.next = t0 + per_xcd + atomicAdd(next, 1u);
         __syncthreads();  // LDS is reused by the next tile
-        tile = sh.next;
+        tile = __builtin_amdgcn_readfirstlane(sh.next);
         __syncthreads();
     }
 }
@@ -679,50 +711,56 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *dst, uint32_t v, siz
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static inline uint32_t blocks(size_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+static inline uint32_t blocks_of(size_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
 template <uint32_t SH>
-static void launch_setup_t(const FrameParams &p, hipStream_t s) {
-    const size_t n = std::max<size_t>(p.n_tris, (size_t)p.n_instances * p.n_vertices);
-    hipLaunchKernelGGL(k_setup<SH>, dim3(blocks(n, 256)), dim3(256), 0, s, p);
+static void launch_setup_t(const FrameParams &p, hipStream_t s, bool batched, uint32_t blocks) {
+    if (batched) {
+        hipLaunchKernelGGL((k_setup<SH, true>), dim3(blocks), dim3(256), 0, s, p);
+    } else {
+        const size_t n = std::max<size_t>(p.v0.n_tris, (size_t)p.v0.n_instances * p.v0.n_vertices);
+        hipLaunchKernelGGL((k_setup<SH, false>), dim3(blocks_of(n, 256)), dim3(256), 0, s, p);
+    }
 }
 
-hipError_t launch_setup(const FrameParams &p, hipStream_t s) {
+hipError_t launch_setup(const FrameParams &p, hipStream_t s, bool batched, uint32_t blocks) {
+    if (batched && blocks == 0) return hipSuccess;
     switch (p.shader) {
-    case FLAT: launch_setup_t<FLAT>(p, s); break;
-    case COLOR: launch_setup_t<COLOR>(p, s); break;
-    case GOURAUD: launch_setup_t<GOURAUD>(p, s); break;
-    case PHONG: launch_setup_t<PHONG>(p, s); break;
-    case DEPTH_ONLY: launch_setup_t<DEPTH_ONLY>(p, s); break;
-    case TEXTURED: launch_setup_t<TEXTURED>(p, s); break;
-    case PHONG_SHADOW: launch_setup_t<PHONG_SHADOW>(p, s); break;
-    case TEX_SHADOW: launch_setup_t<TEX_SHADOW>(p, s); break;
+    case FLAT: launch_setup_t<FLAT>(p, s, batched, blocks); break;
+    case COLOR: launch_setup_t<COLOR>(p, s, batched, blocks); break;
+    case GOURAUD: launch_setup_t<GOURAUD>(p, s, batched, blocks); break;
+    case PHONG: launch_setup_t<PHONG>(p, s, batched, blocks); break;
+    case DEPTH_ONLY: launch_setup_t<DEPTH_ONLY>(p, s, batched, blocks); break;
+    case TEXTURED: launch_setup_t<TEXTURED>(p, s, batched, blocks); break;
+    case PHONG_SHADOW: launch_setup_t<PHONG_SHADOW>(p, s, batched, blocks); break;
+    case TEX_SHADOW: launch_setup_t<TEX_SHADOW>(p, s, batched, blocks); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 template <uint32_t SH>
-static void launch_raster_t(const FrameParams &p, hipStream_t s, uint32_t groups) {
-    hipLaunchKernelGGL(k_raster<SH>, dim3(groups), dim3(256), 0, s, p);
+static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, uint32_t groups) {
+    if (batched) hipLaunchKernelGGL((k_raster<SH, true>), dim3(groups), dim3(256), 0, s, p);
+    else hipLaunchKernelGGL((k_raster<SH, false>), dim3(groups), dim3(256), 0, s, p);
 }
 
 // Workgroups of the tile loop: a multiple of the 8 XCDs, no more than the
 // tiles need (rounded up to 8) and no more than `max_groups`.
-hipError_t launch_raster(const FrameParams &p, hipStream_t s, uint32_t max_groups) {
-    const uint32_t ntiles = p.ntx * p.nty;
-    uint32_t g = ((ntiles + 7u) / 8u) * 8u;
+hipError_t launch_raster(const FrameParams &p, hipStream_t s, bool batched, uint32_t max_groups) {
+    if (p.ntiles == 0) return hipSuccess;
+    uint32_t g = ((p.ntiles + 7u) / 8u) * 8u;
     const uint32_t cap = std::max<uint32_t>(8u, (max_groups / 8u) * 8u);
     if (g > cap) g = cap;
     switch (p.shader) {
-    case FLAT: launch_raster_t<FLAT>(p, s, g); break;
-    case COLOR: launch_raster_t<COLOR>(p, s, g); break;
-    case GOURAUD: launch_raster_t<GOURAUD>(p, s, g); break;
-    case PHONG: launch_raster_t<PHONG>(p, s, g); break;
-    case DEPTH_ONLY: launch_raster_t<DEPTH_ONLY>(p, s, g); break;
-    case TEXTURED: launch_raster_t<TEXTURED>(p, s, g); break;
-    case PHONG_SHADOW: launch_raster_t<PHONG_SHADOW>(p, s, g); break;
-    case TEX_SHADOW: launch_raster_t<TEX_SHADOW>(p, s, g); break;
+    case FLAT: launch_raster_t<FLAT>(p, s, batched, g); break;
+    case COLOR: launch_raster_t<COLOR>(p, s, batched, g); break;
+    case GOURAUD: launch_raster_t<GOURAUD>(p, s, batched, g); break;
+    case PHONG: launch_raster_t<PHONG>(p, s, batched, g); break;
+    case DEPTH_ONLY: launch_raster_t<DEPTH_ONLY>(p, s, batched, g); break;
+    case TEXTURED: launch_raster_t<TEXTURED>(p, s, batched, g); break;
+    case PHONG_SHADOW: launch_raster_t<PHONG_SHADOW>(p, s, batched, g); break;
+    case TEX_SHADOW: launch_raster_t<TEX_SHADOW>(p, s, batched, g); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -730,7 +768,7 @@ hipError_t launch_raster(const FrameParams &p, hipStream_t s, uint32_t max_group
 
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks(n, 256)), dim3(256), 0, s, dst, value, n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_of(n, 256)), dim3(256), 0, s, dst, value, n);
     return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -58,6 +59,8 @@ struct euchip_ctx {
     uint32_t max_groups = 2048; // resident raster workgroups: CUs x 8
     Counters *counters = nullptr;
     Counters *host_counters = nullptr;
+    size_t bytes_in_use = 0;    // device bytes held through this context
+    size_t budget = 0;          // 0: no limit
 };
 
 struct euchip_buffer {
@@ -73,6 +76,50 @@ struct euchip_mesh {
     float *pos = nullptr, *nrm = nullptr, *col = nullptr, *uv = nullptr, *models = nullptr;
     uint32_t *rgba8 = nullptr, *idx = nullptr;
     uint32_t pos_comps = 3, n_vertices = 0, count = 0, n_instances = 1;
+    size_t bytes = 0;
+};
+
+namespace {
+
+// Host half of render() for one request (defined below).
+struct Plan {
+    View v;
+    bool render = false;
+    euchip_status status = EUCHIP_OK;
+    bool clear_pixel = false, clear_depth = false;  // clears left to fill kernels
+    uint64_t n_tris = 0, n_sv = 0;
+};
+
+}  // namespace
+
+struct euchip_batch {
+    euchip_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    euchip_pipeline_desc desc{};
+    std::vector<euchip_request> reqs;
+    std::vector<std::array<float, kUniforms>> uniforms;
+    std::vector<Plan> plans;
+    std::vector<uint32_t> active;       // requests whose render() runs kernels
+    uint32_t tile_sx = 5;
+    std::vector<uint32_t> caps[3];      // bin slots per tile, per tile width (sx - 3) and active request
+    bool tables_dirty = true;
+    // device tables and scratch
+    DevArray<View> views;
+    DevArray<uint2> blk;
+    DevArray<uint32_t> tile_req;
+    DevArray<float4> clip;
+    DevArray<float> vary;
+    DevArray<uint32_t> tile_count, bins, ctrbuf;  // ctrbuf: Counters, then req_max per active request
+    DevArray<uint2> ovf;
+    uint32_t *host_ctr = nullptr;       // pinned copy of ctrbuf
+    size_t host_ctr_words = 0;
+    FrameParams p{};
+    uint32_t setup_blocks = 0;
+    uint64_t tiles_total = 0, bins_total = 0, sv_total = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    bool launched = false;
 };
 
 namespace {
@@ -99,21 +146,52 @@ euchip_status hip_fail(euchip_ctx *ctx, hipError_t e, const char *what) {
         if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
     } while (0)
 
+// Device allocations of a context, checked against its budget.
+euchip_status dev_alloc(euchip_ctx *ctx, void **ptr, size_t bytes, const char *what) {
+    *ptr = nullptr;
+    if (bytes == 0) return EUCHIP_OK;
+    if (ctx->budget && ctx->bytes_in_use + bytes > ctx->budget)
+        return fail(ctx, EUCHIP_OVER_BUDGET, "%s: %zu bytes would exceed the device budget (%zu of %zu in use)", what,
+                    bytes, ctx->bytes_in_use, ctx->budget);
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return hip_fail(ctx, e, what);
+    }
+    ctx->bytes_in_use += bytes;
+    return EUCHIP_OK;
+}
+
+void dev_free(euchip_ctx *ctx, void *ptr, size_t bytes) {
+    if (!ptr) return;
+    (void)hipFree(ptr);
+    ctx->bytes_in_use -= std::min(bytes, ctx->bytes_in_use);
+}
+
+// Grow `a` to at least n elements (contents are not kept).  `s` is the stream
+// that may still use the old storage.
 template <class T>
-euchip_status ensure(euchip_ctx *ctx, DevArray<T> &a, size_t n, const char *what) {
+euchip_status ensure(euchip_ctx *ctx, DevArray<T> &a, size_t n, const char *what, hipStream_t s) {
     if (a.cap >= n && a.ptr) return EUCHIP_OK;
     size_t cap = std::max<size_t>(n, a.cap + a.cap / 2);
     cap = std::max<size_t>(cap, 256);
     if (a.ptr) {
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        HIP_TRY(ctx, hipFree(a.ptr));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        dev_free(ctx, a.ptr, a.cap * sizeof(T));
         a.ptr = nullptr;
         a.cap = 0;
     }
-    hipError_t e = hipMalloc((void **)&a.ptr, cap * sizeof(T));
-    if (e != hipSuccess) return hip_fail(ctx, e, what);
+    euchip_status st = dev_alloc(ctx, (void **)&a.ptr, cap * sizeof(T), what);
+    if (st) return st;
     a.cap = cap;
     return EUCHIP_OK;
+}
+
+template <class T>
+void release(euchip_ctx *ctx, DevArray<T> &a) {
+    dev_free(ctx, a.ptr, a.cap * sizeof(T));
+    a.ptr = nullptr;
+    a.cap = 0;
 }
 
 hipEvent_t take_event(euchip_ctx *ctx) {
@@ -127,26 +205,36 @@ hipEvent_t take_event(euchip_ctx *ctx) {
     return e;
 }
 
-// Launch one kernel, bracketed by events when profiling.
+// Launch one kernel on `s`, bracketed by events on `s` when profiling.
 template <class F>
-euchip_status timed(euchip_ctx *ctx, const char *name, F launch) {
+euchip_status timed_on(euchip_ctx *ctx, hipStream_t s, const char *name, F launch) {
     hipEvent_t a = nullptr, b = nullptr;
     if (ctx->profiling) {
         a = take_event(ctx);
         b = take_event(ctx);
-        if (a) (void)hipEventRecord(a, ctx->stream);
+        if (a) (void)hipEventRecord(a, s);
     }
     hipError_t e = launch();
     if (e != hipSuccess) return hip_fail(ctx, e, name);
     if (ctx->profiling && a && b) {
-        (void)hipEventRecord(b, ctx->stream);
+        (void)hipEventRecord(b, s);
         ctx->pending.push_back({name, a, b});
     }
     return EUCHIP_OK;
 }
 
+template <class F>
+euchip_status timed(euchip_ctx *ctx, const char *name, F launch) {
+    return timed_on(ctx, ctx->stream, name, launch);
+}
+
 void resolve_pending(euchip_ctx *ctx) {
+    std::vector<Pending> keep;  // events of launches still in flight (another stream)
     for (auto &p : ctx->pending) {
+        if (hipEventQuery(p.b) == hipErrorNotReady) {
+            keep.push_back(p);
+            continue;
+        }
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
             Stat &s = ctx->stats[p.name];
@@ -156,35 +244,60 @@ void resolve_pending(euchip_ctx *ctx) {
         ctx->event_pool.push_back(p.a);
         ctx->event_pool.push_back(p.b);
     }
-    ctx->pending.clear();
+    ctx->pending.swap(keep);
 }
 
 bool is_rgba8(const euchip_buffer *b) { return b && b->fmt == EUCHIP_FORMAT_RGBA8; }
 bool is_f32(const euchip_buffer *b) { return b && b->fmt == EUCHIP_FORMAT_F32; }
 
-euchip_status clear_buffer(euchip_ctx *ctx, euchip_buffer *b, uint32_t bits) {
+euchip_status clear_buffer(euchip_ctx *ctx, euchip_buffer *b, uint32_t bits, hipStream_t s) {
     size_t n = (size_t)b->w * b->h;
-    return timed(ctx, "clear", [&] { return launch_fill_u32((uint32_t *)b->ptr, bits, n, ctx->stream); });
+    return timed(ctx, "clear", [&] { return launch_fill_u32((uint32_t *)b->ptr, bits, n, s); });
 }
 
-// The frame: target selection and early-outs of render (pipeline.rs:256-299),
-// band arithmetic of render_par (pipeline.rs:323-330), kernel enqueue.
-euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const float *uniforms,
-                          const euchip_mesh *mesh, euchip_buffer *pixel, euchip_buffer *depth, bool sync) {
-    if (!ctx || !d || !uniforms || !mesh) return fail(ctx, EUCHIP_INVALID_ARG, "null argument");
-    if (mesh->ctx != ctx) return fail(ctx, EUCHIP_INVALID_ARG, "mesh belongs to another context");
+uint32_t f32_bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+
+// Pipeline state shared by every request of a launch.
+void fill_pipe(FrameParams &p, const euchip_pipeline_desc *d) {
+    p.shader = d->shader, p.xform = d->xform, p.pixel_write = d->pixel_write ? 1u : 0u;
+    p.depth_test = d->depth_test, p.depth_write = d->depth_write ? 1u : 0u, p.y_up = d->y_up ? 1u : 0u;
+    p.z_clip = d->z_clip ? 1u : 0u, p.z_min = d->z_min, p.z_max = d->z_max;
+    p.cull = d->cull, p.msaa_level = d->aa_msaa ? std::min<uint32_t>(d->aa_level, 6u) : 0u;
+    p.band_par = d->band_policy == EUCHIP_BANDS_PAR ? 1u : 0u;
+}
+
+euchip_status check_desc(euchip_ctx *ctx, const euchip_pipeline_desc *d) {
     if (d->shader > EUCHIP_SHADER_TEX_SHADOW) return fail(ctx, EUCHIP_INVALID_ARG, "unknown shader %u", d->shader);
     if (d->depth_test > EUCHIP_DEPTH_GREATER || d->cull > EUCHIP_CULL_FRONT || d->band_policy > EUCHIP_BANDS_SEQ ||
         d->xform > EUCHIP_XFORM_MVP)
         return fail(ctx, EUCHIP_INVALID_ARG, "enum field out of range");
+    return EUCHIP_OK;
+}
+
+// The host half of render() for one request (pipeline.rs:256-299, 323-330):
+// argument checks, target selection, the size assertion, the early-outs and
+// render_par's band height.  `status` carries the error render() reports after
+// its clears (size mismatch, zero band height); `render` says whether kernels
+// run; clears that cannot be fused into the raster pass are flagged.
+euchip_status plan_request(euchip_ctx *ctx, const euchip_pipeline_desc *d, const float *uniforms,
+                           const euchip_mesh *mesh, euchip_buffer *pixel, euchip_buffer *depth,
+                           const euchip_buffer *texture, const euchip_buffer *shadow, Plan &P) {
+    P = Plan();
+    memset(&P.v, 0, sizeof(P.v));
+    if (!uniforms || !mesh) return fail(ctx, EUCHIP_INVALID_ARG, "null argument");
+    if (mesh->ctx != ctx) return fail(ctx, EUCHIP_INVALID_ARG, "mesh belongs to another context");
     if (pixel && (!is_rgba8(pixel) || pixel->ctx != ctx)) return fail(ctx, EUCHIP_INVALID_ARG, "pixel target must be RGBA8");
     if (depth && (!is_f32(depth) || depth->ctx != ctx)) return fail(ctx, EUCHIP_INVALID_ARG, "depth target must be f32");
     const uint32_t sh = d->shader;
     if ((sh == EUCHIP_SHADER_TEXTURED || sh == EUCHIP_SHADER_TEX_SHADOW) &&
-        (!is_rgba8(d->texture) || d->texture->w == 0 || d->texture->h == 0))
+        (!is_rgba8(texture) || texture->w == 0 || texture->h == 0))
         return fail(ctx, EUCHIP_INVALID_ARG, "Linearly-interpolated texture cannot have no size");
     if ((sh == EUCHIP_SHADER_PHONG_SHADOW || sh == EUCHIP_SHADER_TEX_SHADOW) &&
-        (!is_f32(d->shadow) || d->shadow->w == 0 || d->shadow->h == 0))
+        (!is_f32(shadow) || shadow->w == 0 || shadow->h == 0))
         return fail(ctx, EUCHIP_INVALID_ARG, "Linearly-interpolated texture cannot have no size");
     if ((sh == EUCHIP_SHADER_FLAT && !mesh->rgba8) || (sh == EUCHIP_SHADER_COLOR && !mesh->col) ||
         ((sh == EUCHIP_SHADER_GOURAUD || sh == EUCHIP_SHADER_PHONG || sh == EUCHIP_SHADER_PHONG_SHADOW ||
@@ -193,142 +306,387 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         return fail(ctx, EUCHIP_INVALID_ARG, "mesh lacks an attribute the shader reads");
 
     const bool uses_depth = d->depth_test != EUCHIP_DEPTH_NONE || d->depth_write;
-    uint32_t pw = pixel ? pixel->w : 0, ph = pixel ? pixel->h : 0;
-    uint32_t dw = depth ? depth->w : 0, dh = depth ? depth->h : 0;
-
-    // Clears requested through the load ops happen first, whatever render does.
-    bool fuse_color = false, fuse_depth = false;
+    const uint32_t pw = pixel ? pixel->w : 0, ph = pixel ? pixel->h : 0;
+    const uint32_t dw = depth ? depth->w : 0, dh = depth ? depth->h : 0;
     uint32_t W = 0, H = 0;
-    bool render_nothing = false;
+    bool nothing = false;
     if (!d->pixel_write && !uses_depth) {
-        render_nothing = true;
+        nothing = true;
     } else if (d->pixel_write && !uses_depth) {
         W = pw, H = ph;
     } else if (!d->pixel_write && uses_depth) {
         W = dw, H = dh;
+    } else if (pw != dw || ph != dh) {
+        P.status = fail(ctx, EUCHIP_SIZE_MISMATCH, "Pixel target size is compatible with depth target size");
+        nothing = true;
     } else {
-        if (pw != dw || ph != dh) {
-            if (d->clear_color_enable && pixel) {
-                euchip_status s = clear_buffer(ctx, pixel, d->clear_color);
-                if (s) return s;
-            }
-            if (d->clear_depth_enable && depth) {
-                uint32_t bits;
-                memcpy(&bits, &d->clear_depth, 4);
-                euchip_status s = clear_buffer(ctx, depth, bits);
-                if (s) return s;
-            }
-            return fail(ctx, EUCHIP_SIZE_MISMATCH, "Pixel target size is compatible with depth target size");
-        }
         W = pw, H = ph;
     }
     const uint32_t L = d->aa_msaa ? std::min<uint32_t>(d->aa_level, 6u) : 0u;
     uint32_t G = H;
-    euchip_status band_status = EUCHIP_OK;
-    if (!render_nothing && d->band_policy == EUCHIP_BANDS_PAR) {
+    if (!nothing && d->band_policy == EUCHIP_BANDS_PAR) {
         G = (uint32_t)((20000ull << L) / std::max<uint32_t>(W, 1u));
         if (G == 0) {
-            band_status = fail(ctx, EUCHIP_BAND_ZERO, "attempt to divide by zero (render_par group_rows == 0)");
-            render_nothing = true;
-        } else {
-            uint32_t threads = std::max<uint32_t>(d->threads, 1u);
-            if (std::min<uint32_t>(H / G, threads) == 0) render_nothing = true;  // no worker threads spawned
+            P.status = fail(ctx, EUCHIP_BAND_ZERO, "attempt to divide by zero (render_par group_rows == 0)");
+            nothing = true;
+        } else if (std::min<uint32_t>(H / G, std::max<uint32_t>(d->threads, 1u)) == 0) {
+            nothing = true;  // no worker threads spawned
         }
     }
-    const uint64_t total = (uint64_t)mesh->count * mesh->n_instances;
-    const uint64_t n_tris = total / 3;
-    if (W == 0 || H == 0 || n_tris == 0) render_nothing = true;
-    if (!render_nothing) {
-        fuse_color = d->clear_color_enable && pixel && pixel->w == W && pixel->h == H;
-        fuse_depth = d->clear_depth_enable && depth && depth->w == W && depth->h == H;
-    }
-    if (d->clear_color_enable && pixel && !fuse_color) {
-        euchip_status s = clear_buffer(ctx, pixel, d->clear_color);
-        if (s) return s;
-    }
-    if (d->clear_depth_enable && depth && !fuse_depth) {
-        uint32_t bits;
-        memcpy(&bits, &d->clear_depth, 4);
-        euchip_status s = clear_buffer(ctx, depth, bits);
-        if (s) return s;
-    }
-    if (render_nothing) {
-        if (sync) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        return band_status;
-    }
-    if (W >= (1u << 24) || H >= (1u << 24)) return fail(ctx, EUCHIP_INVALID_ARG, "target larger than 2^24");
+    P.n_tris = (uint64_t)mesh->count * mesh->n_instances / 3;
+    P.n_sv = (uint64_t)mesh->n_instances * mesh->n_vertices;
+    if (W == 0 || H == 0 || P.n_tris == 0) nothing = true;
+    if (!nothing && (W >= (1u << 24) || H >= (1u << 24))) return fail(ctx, EUCHIP_INVALID_ARG, "target larger than 2^24");
+    const bool fuse_color = !nothing && d->clear_color_enable && pixel && pixel->w == W && pixel->h == H;
+    const bool fuse_depth = !nothing && d->clear_depth_enable && depth && depth->w == W && depth->h == H;
+    P.clear_pixel = d->clear_color_enable && pixel && !fuse_color;
+    P.clear_depth = d->clear_depth_enable && depth && !fuse_depth;
+    P.render = !nothing;
+    if (!P.render) return EUCHIP_OK;
 
+    View &v = P.v;
+    v.W = W, v.H = H, v.G = G;
+    // Targets that are not the render size are never touched (Empty-like use).
+    v.pixel = pixel && pixel->w == W && pixel->h == H ? (uint32_t *)pixel->ptr : nullptr;
+    v.depth = depth && depth->w == W && depth->h == H ? (float *)depth->ptr : nullptr;
+    if (uses_depth && !v.depth) return fail(ctx, EUCHIP_INVALID_ARG, "depth test without a depth target");
+    v.clear_color_en = fuse_color ? 1u : 0u, v.clear_color = d->clear_color;
+    v.clear_depth_en = fuse_depth ? 1u : 0u, v.clear_depth = d->clear_depth;
+    v.ndc_x = 2.0f / (float)W;
+    v.ndc_y = -2.0f / (float)H;
+    v.pos = mesh->pos, v.nrm = mesh->nrm, v.col = mesh->col, v.uv = mesh->uv, v.rgba8 = mesh->rgba8;
+    v.idx = mesh->idx, v.models = mesh->models;
+    v.pos_comps = mesh->pos_comps, v.count = mesh->count, v.n_vertices = mesh->n_vertices;
+    v.n_instances = mesh->n_instances, v.n_tris = (uint32_t)P.n_tris;
+    if (texture) v.tex = (const uint32_t *)texture->ptr, v.tex_w = texture->w, v.tex_h = texture->h;
+    if (shadow) v.shadow = (const float *)shadow->ptr, v.sh_w = shadow->w, v.sh_h = shadow->h;
+    memcpy(v.u, uniforms, sizeof(v.u));
+    return EUCHIP_OK;
+}
+
+// Tiles of a W x H target with tile_sx (tiles of 1 << sx by 256 >> sx pixels).
+void tile_grid(View &v, uint32_t sx) {
+    const uint32_t tw = 1u << sx, th = 256u >> sx;
+    v.ntx = (v.W + tw - 1) / tw;
+    v.nty = (v.H + th - 1) / th;
+}
+
+euchip_status enqueue_clears(euchip_ctx *ctx, const euchip_pipeline_desc *d, const Plan &P, euchip_buffer *pixel,
+                             euchip_buffer *depth, hipStream_t s) {
+    euchip_status st;
+    if (P.clear_pixel && (st = clear_buffer(ctx, pixel, d->clear_color, s))) return st;
+    if (P.clear_depth && (st = clear_buffer(ctx, depth, f32_bits(d->clear_depth), s))) return st;
+    return EUCHIP_OK;
+}
+
+// Read back the counters after a frame and grow the bins; true when the frame
+// was dropped because the overflow list filled up.
+euchip_status after_frame(euchip_ctx *ctx, bool &dropped) {
+    const Counters &hc = *ctx->host_counters;
+    if (hc.max_count > ctx->tile_cap) {  // grow the bins for the frames that follow
+        uint32_t c = ctx->tile_cap;
+        while (c < hc.max_count && c < (1u << 24)) c <<= 1;
+        ctx->tile_cap = c;
+    }
+    dropped = (hc.flags & FLAG_BIN_OVERFLOW) != 0;
+    if (dropped) {
+        HIP_TRY(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(Counters), ctx->stream));
+        HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return EUCHIP_OK;
+}
+
+// render() of one request on the context's stream.
+euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const float *uniforms,
+                          const euchip_mesh *mesh, euchip_buffer *pixel, euchip_buffer *depth, bool sync) {
+    if (!ctx || !d || !uniforms || !mesh) return fail(ctx, EUCHIP_INVALID_ARG, "null argument");
+    euchip_status s = check_desc(ctx, d);
+    if (s) return s;
+    Plan P;
+    if ((s = plan_request(ctx, d, uniforms, mesh, pixel, depth, d->texture, d->shadow, P))) return s;
+    if ((s = enqueue_clears(ctx, d, P, pixel, depth, ctx->stream))) return s;
+    if (!P.render) {
+        if (sync) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return P.status;
+    }
     FrameParams p;
     memset(&p, 0, sizeof(p));
-    p.W = W, p.H = H;
-    p.pixel = pixel ? (uint32_t *)pixel->ptr : nullptr;
-    p.depth = depth ? (float *)depth->ptr : nullptr;
-    // Targets that are not the render size are never touched (Empty-like use).
-    if (pixel && (pixel->w != W || pixel->h != H)) p.pixel = nullptr;
-    if (depth && (depth->w != W || depth->h != H)) p.depth = nullptr;
-    p.shader = sh, p.xform = d->xform, p.pixel_write = d->pixel_write ? 1u : 0u;
-    p.depth_test = d->depth_test, p.depth_write = d->depth_write ? 1u : 0u, p.y_up = d->y_up ? 1u : 0u;
-    p.z_clip = d->z_clip ? 1u : 0u, p.z_min = d->z_min, p.z_max = d->z_max;
-    p.cull = d->cull, p.msaa_level = L, p.band_par = d->band_policy == EUCHIP_BANDS_PAR ? 1u : 0u, p.G = G;
-    p.clear_color_en = fuse_color ? 1u : 0u, p.clear_color = d->clear_color;
-    p.clear_depth_en = fuse_depth ? 1u : 0u, p.clear_depth = d->clear_depth;
-    if (uses_depth && !p.depth) return fail(ctx, EUCHIP_INVALID_ARG, "depth test without a depth target");
-    p.pos = mesh->pos, p.nrm = mesh->nrm, p.col = mesh->col, p.uv = mesh->uv, p.rgba8 = mesh->rgba8;
-    p.idx = mesh->idx, p.models = mesh->models;
-    p.pos_comps = mesh->pos_comps, p.count = mesh->count, p.n_vertices = mesh->n_vertices;
-    p.n_instances = mesh->n_instances, p.n_tris = (uint32_t)n_tris;
-    p.ndc_x = 2.0f / (float)W;
-    p.ndc_y = -2.0f / (float)H;
-    if (d->texture) p.tex = (const uint32_t *)d->texture->ptr, p.tex_w = d->texture->w, p.tex_h = d->texture->h;
-    if (d->shadow) p.shadow = (const float *)d->shadow->ptr, p.sh_w = d->shadow->w, p.sh_h = d->shadow->h;
-    memcpy(p.u, uniforms, sizeof(p.u));
-    p.ntx = (W + kTileW - 1) / kTileW;
-    p.nty = (H + kTileH - 1) / kTileH;
-    const size_t ntiles = (size_t)p.ntx * p.nty;
+    fill_pipe(p, d);
+    p.tile_sx = 5;
+    p.v0 = P.v;
+    tile_grid(p.v0, p.tile_sx);
+    const size_t ntiles = (size_t)p.v0.ntx * p.v0.nty;
+    p.ntiles = (uint32_t)ntiles;
+    p.n_views = 1;
 
-    euchip_status s;
-    const size_t nsv = (size_t)mesh->n_instances * mesh->n_vertices;
-    if ((s = ensure(ctx, ctx->clip, std::max<size_t>(nsv, 1), "clip positions"))) return s;
-    if ((s = ensure(ctx, ctx->vary, std::max<size_t>(nsv * vd_count(sh), 1), "vertex data"))) return s;
+    const size_t nsv = P.n_sv;
+    if ((s = ensure(ctx, ctx->clip, std::max<size_t>(nsv, 1), "clip positions", ctx->stream))) return s;
+    if ((s = ensure(ctx, ctx->vary, std::max<size_t>(nsv * vd_count(d->shader), 1), "vertex data", ctx->stream)))
+        return s;
     const bool fresh_tiles = ctx->tile_count.cap < ntiles;
-    if ((s = ensure(ctx, ctx->tile_count, ntiles, "tile counts"))) return s;
+    if ((s = ensure(ctx, ctx->tile_count, ntiles, "tile counts", ctx->stream))) return s;
     if (fresh_tiles) HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4, ctx->stream));
-    if ((s = ensure(ctx, ctx->ovf, std::max<size_t>((size_t)1 << 20, (size_t)n_tris), "overflow list"))) return s;
+    if ((s = ensure(ctx, ctx->ovf, std::max<size_t>((size_t)1 << 20, (size_t)P.n_tris), "overflow list", ctx->stream)))
+        return s;
 
     for (int attempt = 0;; attempt++) {
-        if ((s = ensure(ctx, ctx->bins, ntiles * ctx->tile_cap, "bins"))) return s;
+        if ((s = ensure(ctx, ctx->bins, ntiles * ctx->tile_cap, "bins", ctx->stream))) return s;
         p.clip = ctx->clip.ptr;
         p.vary = ctx->vary.ptr;
         p.ctr = ctx->counters;
+        p.req_max = &ctx->counters->max_count;
         p.tile_count = ctx->tile_count.ptr;
         p.bins = ctx->bins.ptr;
-        p.tile_cap = ctx->tile_cap;
+        p.v0.tile_cap = ctx->tile_cap;
+        p.v0.bin_base = 0;
         p.ovf = ctx->ovf.ptr;
         p.ovf_cap = (uint32_t)std::min<size_t>(ctx->ovf.cap, 0xFFFFFFF0ull);
         p.frame = ctx->frame++;
-        if ((s = timed(ctx, "setup", [&] { return launch_setup(p, ctx->stream); }))) return s;
-        if ((s = timed(ctx, "raster", [&] { return launch_raster(p, ctx->stream, ctx->max_groups); }))) return s;
+        if ((s = timed(ctx, "setup", [&] { return launch_setup(p, ctx->stream, false, 0); }))) return s;
+        if ((s = timed(ctx, "raster", [&] { return launch_raster(p, ctx->stream, false, ctx->max_groups); }))) return s;
         if (!sync) return EUCHIP_OK;
         HIP_TRY(ctx, hipMemcpyAsync(ctx->host_counters, ctx->counters, sizeof(Counters), hipMemcpyDeviceToHost,
                                     ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->profiling) resolve_pending(ctx);
-        const Counters &hc = *ctx->host_counters;
-        if (hc.max_count > ctx->tile_cap) {  // grow the bins for the frames that follow
-            uint32_t c = ctx->tile_cap;
-            while (c < hc.max_count && c < (1u << 24)) c <<= 1;
-            ctx->tile_cap = c;
-        }
-        if (!(hc.flags & FLAG_BIN_OVERFLOW)) return EUCHIP_OK;
-        // The overflow list filled up: nothing was written.  Reset and retry larger.
-        HIP_TRY(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(Counters), ctx->stream));
-        HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4, ctx->stream));
+        bool dropped = false;
+        if ((s = after_frame(ctx, dropped))) return s;
+        if (!dropped) return EUCHIP_OK;
+        // The overflow list filled up: nothing was written.  Retry larger.
         if (attempt >= 3) return fail(ctx, EUCHIP_OVER_BUDGET, "bin storage overflow");
-        if ((s = ensure(ctx, ctx->ovf, ctx->ovf.cap * 4, "overflow list"))) return s;
+        if ((s = ensure(ctx, ctx->ovf, ctx->ovf.cap * 4, "overflow list", ctx->stream))) return s;
     }
 }
 
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+
+uint32_t pow2_at_least(uint64_t n) {
+    uint32_t c = 16;
+    while (c < n && c < (1u << 24)) c <<= 1;
+    return c;
+}
+
+// First guess of a request's bin slots per tile: twice its triangles spread
+// evenly over its tiles, at least 16, at most all of them.
+uint32_t initial_cap(uint64_t n_tris, uint64_t ntiles) {
+    const uint64_t even = ntiles ? (2 * n_tris + ntiles - 1) / ntiles : n_tris;
+    return pow2_at_least(std::min<uint64_t>(std::max<uint64_t>(even, 16), std::max<uint64_t>(n_tris, 16)));
+}
+
+void drop_graph(euchip_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Lay the active requests out in the launch's tile, vertex and bin spaces for
+// the current tile width and upload the tables.
+euchip_status build_tables(euchip_batch *b) {
+    euchip_ctx *ctx = b->ctx;
+    const uint32_t sx = b->tile_sx;
+    std::vector<uint32_t> &caps = b->caps[sx - 3];
+    const size_t na = b->active.size();
+    std::vector<View> hv(na);
+    std::vector<uint2> hblk;
+    std::vector<uint32_t> htile;
+    uint64_t tiles = 0, bins = 0, sv = 0, tris = 0;
+    for (size_t k = 0; k < na; k++) {
+        const Plan &P = b->plans[b->active[k]];
+        View v = P.v;
+        tile_grid(v, sx);
+        const uint64_t nt = (uint64_t)v.ntx * v.nty;
+        if (caps.size() < na) caps.resize(na, 0);
+        if (caps[k] == 0) caps[k] = initial_cap(P.n_tris, nt);
+        v.tile_base = (uint32_t)tiles;
+        v.sv_base = (uint32_t)sv;
+        v.tile_cap = caps[k];
+        v.bin_base = bins;
+        hv[k] = v;
+        const uint64_t nb = (std::max<uint64_t>(P.n_sv, P.n_tris) + 255) / 256;
+        for (uint64_t j = 0; j < nb; j++) hblk.push_back(make_uint2((uint32_t)k, (uint32_t)j));
+        htile.insert(htile.end(), nt, (uint32_t)k);
+        tiles += nt;
+        bins += nt * caps[k];
+        sv += P.n_sv;
+        tris += P.n_tris;
+        if (tiles >= (1ull << 32) || sv >= (1ull << 32) || hblk.size() >= (1ull << 31))
+            return fail(ctx, EUCHIP_INVALID_ARG, "batch too large for 32-bit tile / vertex ids");
+    }
+    euchip_status s;
+    const uint32_t vdc = vd_count(b->desc.shader);
+    if ((s = ensure(ctx, b->views, std::max<size_t>(na, 1), "batch views", b->stream))) return s;
+    if ((s = ensure(ctx, b->blk, std::max<size_t>(hblk.size(), 1), "batch blocks", b->stream))) return s;
+    if ((s = ensure(ctx, b->tile_req, std::max<size_t>(tiles, 1), "batch tiles", b->stream))) return s;
+    if ((s = ensure(ctx, b->clip, std::max<size_t>(sv, 1), "batch clip positions", b->stream))) return s;
+    if ((s = ensure(ctx, b->vary, std::max<size_t>(sv * vdc, 1), "batch vertex data", b->stream))) return s;
+    if ((s = ensure(ctx, b->tile_count, std::max<size_t>(tiles, 1), "batch tile counts", b->stream))) return s;
+    if ((s = ensure(ctx, b->bins, std::max<size_t>(bins, 1), "batch bins", b->stream))) return s;
+    if ((s = ensure(ctx, b->ovf, std::max<size_t>((size_t)1 << 20, tris), "batch overflow list", b->stream))) return s;
+    const size_t words = sizeof(Counters) / 4 + na;
+    if ((s = ensure(ctx, b->ctrbuf, words, "batch counters", b->stream))) return s;
+    if (b->host_ctr_words < words) {
+        if (b->host_ctr) (void)hipHostFree(b->host_ctr);
+        b->host_ctr = nullptr;
+        b->host_ctr_words = 0;
+        HIP_TRY(ctx, hipHostMalloc((void **)&b->host_ctr, words * 4, hipHostMallocDefault));
+        b->host_ctr_words = words;
+    }
+    memset(b->host_ctr, 0, words * 4);
+    HIP_TRY(ctx, hipMemcpyAsync(b->views.ptr, hv.data(), na * sizeof(View), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(b->blk.ptr, hblk.data(), hblk.size() * sizeof(uint2), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(b->tile_req.ptr, htile.data(), htile.size() * 4, hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(ctx, hipMemsetAsync(b->tile_count.ptr, 0, b->tile_count.cap * 4, b->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(b->stream));  // the host tables are freed on return
+
+    FrameParams &p = b->p;
+    memset(&p, 0, sizeof(p));
+    fill_pipe(p, &b->desc);
+    p.clip = b->clip.ptr;
+    p.vary = b->vary.ptr;
+    p.ctr = (Counters *)b->ctrbuf.ptr;
+    p.req_max = b->ctrbuf.ptr + sizeof(Counters) / 4;
+    p.tile_count = b->tile_count.ptr;
+    p.bins = b->bins.ptr;
+    p.ovf = b->ovf.ptr;
+    p.ovf_cap = (uint32_t)std::min<size_t>(b->ovf.cap, 0xFFFFFFF0ull);
+    p.frame = 0;
+    p.ntiles = (uint32_t)tiles;
+    p.tile_sx = sx;
+    p.n_views = (uint32_t)na;
+    p.views = b->views.ptr;
+    p.blk = b->blk.ptr;
+    p.tile_req = b->tile_req.ptr;
+    b->setup_blocks = (uint32_t)hblk.size();
+    b->tiles_total = tiles;
+    b->bins_total = bins;
+    b->sv_total = sv;
+    b->tables_dirty = false;
+    drop_graph(b);
+    return EUCHIP_OK;
+}
+
+// The work of one launch, enqueued on the batch's stream (also what a graph
+// captures): leftover clears, counter reset, setup, raster, counter read-back.
+// Eager launches of a profiling context time setup and raster with events.
+euchip_status enqueue_batch(euchip_batch *b, bool capturing = false) {
+    euchip_ctx *ctx = b->ctx;
+    for (size_t i = 0; i < b->reqs.size(); i++) {
+        const Plan &P = b->plans[i];
+        if (P.clear_pixel) HIP_TRY(ctx, launch_fill_u32((uint32_t *)b->reqs[i].pixel->ptr, b->desc.clear_color,
+                                                        (size_t)b->reqs[i].pixel->w * b->reqs[i].pixel->h, b->stream));
+        if (P.clear_depth) HIP_TRY(ctx, launch_fill_u32((uint32_t *)b->reqs[i].depth->ptr, f32_bits(b->desc.clear_depth),
+                                                        (size_t)b->reqs[i].depth->w * b->reqs[i].depth->h, b->stream));
+    }
+    if (b->active.empty()) return EUCHIP_OK;
+    const size_t words = sizeof(Counters) / 4 + b->active.size();
+    HIP_TRY(ctx, launch_fill_u32(b->ctrbuf.ptr, 0u, words, b->stream));
+    euchip_status s;
+    if (ctx->profiling && !capturing) {
+        if ((s = timed_on(ctx, b->stream, "batch_setup", [&] { return launch_setup(b->p, b->stream, true, b->setup_blocks); })))
+            return s;
+        if ((s = timed_on(ctx, b->stream, "batch_raster", [&] { return launch_raster(b->p, b->stream, true, ctx->max_groups); })))
+            return s;
+    } else {
+        HIP_TRY(ctx, launch_setup(b->p, b->stream, true, b->setup_blocks));
+        HIP_TRY(ctx, launch_raster(b->p, b->stream, true, ctx->max_groups));
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(b->host_ctr, b->ctrbuf.ptr, words * 4, hipMemcpyDeviceToHost, b->stream));
+    return EUCHIP_OK;
+}
+
+euchip_status launch_batch(euchip_batch *b, bool use_graph) {
+    euchip_ctx *ctx = b->ctx;
+    euchip_status s;
+    if (b->tables_dirty && (s = build_tables(b))) return s;
+    if (use_graph) {
+        if (!b->exec) {
+            HIP_TRY(ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            s = enqueue_batch(b, true);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (s) {
+                if (g) (void)hipGraphDestroy(g);
+                return s;
+            }
+            if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+            b->graph = g;
+            HIP_TRY(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        }
+        HIP_TRY(ctx, hipGraphLaunch(b->exec, b->stream));
+    } else if ((s = enqueue_batch(b))) {
+        return s;
+    }
+    HIP_TRY(ctx, hipEventRecord(b->done, b->stream));
+    b->launched = true;
+    return EUCHIP_OK;
+}
+
+// Wait for the last launch; grow bins that overflowed; re-run a launch whose
+// overflow list filled up (its targets were not written).
+euchip_status wait_batch(euchip_batch *b) {
+    euchip_ctx *ctx = b->ctx;
+    for (int attempt = 0;; attempt++) {
+        HIP_TRY(ctx, hipStreamSynchronize(b->stream));
+        if (ctx->profiling) {
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            resolve_pending(ctx);
+        }
+        if (!b->launched || b->active.empty()) return EUCHIP_OK;
+        const Counters &hc = *(const Counters *)b->host_ctr;
+        const uint32_t *req_max = b->host_ctr + sizeof(Counters) / 4;
+        std::vector<uint32_t> &caps = b->caps[b->tile_sx - 3];
+        for (size_t k = 0; k < b->active.size(); k++) {
+            if (req_max[k] > caps[k]) {
+                caps[k] = pow2_at_least(req_max[k]);
+                b->tables_dirty = true;
+            }
+        }
+        if (!(hc.flags & FLAG_BIN_OVERFLOW)) return EUCHIP_OK;
+        if (attempt >= 3) return fail(ctx, EUCHIP_OVER_BUDGET, "batch bin storage overflow");
+        euchip_status s;
+        if ((s = ensure(ctx, b->ovf, b->ovf.cap * 4, "batch overflow list", b->stream))) return s;
+        b->tables_dirty = true;  // rebuild also zeroes the tile counters the aborted raster left
+        if ((s = launch_batch(b, false))) return s;
+    }
+}
+
+void free_batch_memory(euchip_batch *b) {
+    euchip_ctx *ctx = b->ctx;
+    drop_graph(b);
+    release(ctx, b->views);
+    release(ctx, b->blk);
+    release(ctx, b->tile_req);
+    release(ctx, b->clip);
+    release(ctx, b->vary);
+    release(ctx, b->tile_count);
+    release(ctx, b->bins);
+    release(ctx, b->ctrbuf);
+    release(ctx, b->ovf);
+    if (b->host_ctr) (void)hipHostFree(b->host_ctr);
+    b->host_ctr = nullptr;
+    b->host_ctr_words = 0;
+}
+
 }  // namespace
+
+namespace euchip {
+
+size_t request_scratch_estimate(const euchip_pipeline_desc *d, const euchip_mesh *m, uint32_t W, uint32_t H) {
+    const uint64_t n_tris = (uint64_t)m->count * m->n_instances / 3;
+    const uint64_t n_sv = (uint64_t)m->n_instances * m->n_vertices;
+    const uint64_t nt = (uint64_t)((W + 31) / 32) * ((H + 7) / 8);
+    const uint64_t nb = (std::max<uint64_t>(n_sv, n_tris) + 255) / 256;
+    return sizeof(View) + nb * sizeof(uint2) + nt * 8 + n_sv * (16 + 4 * vd_count(d->shader)) +
+           nt * initial_cap(n_tris, nt) * 4 + 4 + 8 * n_tris;
+}
+
+size_t batch_fixed_bytes() { return sizeof(Counters) + 8 * ((size_t)1 << 20); }
+
+}  // namespace euchip
 
 extern "C" {
 
@@ -379,11 +737,11 @@ euchip_status euchip_ctx_destroy(euchip_ctx *ctx) {
         (void)hipEventDestroy(p.b);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-    (void)hipFree(ctx->clip.ptr);
-    (void)hipFree(ctx->vary.ptr);
-    (void)hipFree(ctx->tile_count.ptr);
-    (void)hipFree(ctx->bins.ptr);
-    (void)hipFree(ctx->ovf.ptr);
+    release(ctx, ctx->clip);
+    release(ctx, ctx->vary);
+    release(ctx, ctx->tile_count);
+    release(ctx, ctx->bins);
+    release(ctx, ctx->ovf);
     (void)hipFree(ctx->counters);
     if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -397,21 +755,23 @@ void *euchip_ctx_stream(euchip_ctx *ctx) { return ctx ? (void *)ctx->stream : nu
 
 euchip_status euchip_synchronize(euchip_ctx *ctx) {
     if (!ctx) return EUCHIP_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
     HIP_TRY(ctx, hipMemcpyAsync(ctx->host_counters, ctx->counters, sizeof(Counters), hipMemcpyDeviceToHost,
                                 ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->profiling) resolve_pending(ctx);
-    const Counters &hc = *ctx->host_counters;
-    if (hc.max_count > ctx->tile_cap) {
-        uint32_t c = ctx->tile_cap;
-        while (c < hc.max_count && c < (1u << 24)) c <<= 1;
-        ctx->tile_cap = c;
-    }
-    if (hc.flags & FLAG_BIN_OVERFLOW) {
-        HIP_TRY(ctx, hipMemset(ctx->counters, 0, sizeof(Counters)));
-        HIP_TRY(ctx, hipMemset(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4));
-        return fail(ctx, EUCHIP_OVER_BUDGET, "bin storage overflow in an asynchronous frame; it was dropped");
-    }
+    bool dropped = false;
+    euchip_status s = after_frame(ctx, dropped);
+    if (s) return s;
+    if (dropped) return fail(ctx, EUCHIP_OVER_BUDGET, "bin storage overflow in an asynchronous frame; it was dropped");
+    return EUCHIP_OK;
+}
+
+size_t euchip_ctx_bytes_in_use(const euchip_ctx *ctx) { return ctx ? ctx->bytes_in_use : 0; }
+
+euchip_status euchip_ctx_set_budget(euchip_ctx *ctx, size_t bytes) {
+    if (!ctx) return EUCHIP_INVALID_ARG;
+    ctx->budget = bytes;
     return EUCHIP_OK;
 }
 
@@ -449,11 +809,8 @@ euchip_status euchip_buffer_create(euchip_ctx *ctx, euchip_format fmt, uint32_t 
         return fail(ctx, EUCHIP_INVALID_ARG, "bad buffer arguments");
     *out = nullptr;
     void *ptr = nullptr;
-    size_t bytes = (size_t)w * h * 4;
-    if (bytes) {
-        hipError_t e = hipMalloc(&ptr, bytes);
-        if (e != hipSuccess) return hip_fail(ctx, e, "buffer");
-    }
+    euchip_status s = dev_alloc(ctx, &ptr, (size_t)w * h * 4, "buffer");
+    if (s) return s;
     *out = new euchip_buffer{ctx, fmt, w, h, ptr, true};
     return EUCHIP_OK;
 }
@@ -469,8 +826,8 @@ euchip_status euchip_buffer_wrap(euchip_ctx *ctx, euchip_format fmt, uint32_t w,
 euchip_status euchip_buffer_destroy(euchip_buffer *b) {
     if (!b) return EUCHIP_INVALID_ARG;
     if (b->owned && b->ptr) {
-        (void)hipStreamSynchronize(b->ctx->stream);
-        (void)hipFree(b->ptr);
+        (void)hipDeviceSynchronize();  // batches on other streams may still use it
+        dev_free(b->ctx, b->ptr, (size_t)b->w * b->h * 4);
     }
     delete b;
     return EUCHIP_OK;
@@ -494,7 +851,7 @@ euchip_status euchip_buffer_clear(euchip_buffer *b, const void *texel) {
     if (!b || !texel) return EUCHIP_INVALID_ARG;
     uint32_t bits;
     memcpy(&bits, texel, 4);
-    return clear_buffer(b->ctx, b, bits);
+    return clear_buffer(b->ctx, b, bits, b->ctx->stream);
 }
 
 void *euchip_buffer_device_ptr(const euchip_buffer *b) { return b ? b->ptr : nullptr; }
@@ -521,11 +878,13 @@ euchip_status euchip_mesh_create(euchip_ctx *ctx, const euchip_mesh_desc *d, euc
     m->n_vertices = (uint32_t)d->n_vertices;
     m->count = (uint32_t)d->count;
     m->n_instances = d->n_instances;
-    auto up = [&](void **dst, const void *src, size_t bytes) -> hipError_t {
-        if (!src || !bytes) return hipSuccess;
-        hipError_t e = hipMalloc(dst, bytes);
-        if (e != hipSuccess) return e;
-        return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    auto up = [&](void **dst, const void *src, size_t bytes) -> euchip_status {
+        if (!src || !bytes) return EUCHIP_OK;
+        euchip_status st = dev_alloc(ctx, dst, bytes, "mesh");
+        if (st) return st;
+        m->bytes += bytes;
+        hipError_t e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        return e == hipSuccess ? EUCHIP_OK : hip_fail(ctx, e, "mesh upload");
     };
     const size_t nv = d->n_vertices;
     std::vector<float> ident;
@@ -536,16 +895,15 @@ euchip_status euchip_mesh_create(euchip_ctx *ctx, const euchip_mesh_desc *d, euc
             for (int k = 0; k < 4; k++) ident[16 * i + 5 * k] = 1.0f;
         models = ident.data();
     }
-    hipError_t e;
-    if ((e = up((void **)&m->pos, d->pos, nv * d->pos_comps * 4)) != hipSuccess ||
-        (e = up((void **)&m->nrm, d->nrm, nv * 12)) != hipSuccess ||
-        (e = up((void **)&m->col, d->col, nv * 16)) != hipSuccess ||
-        (e = up((void **)&m->uv, d->uv, nv * 8)) != hipSuccess ||
-        (e = up((void **)&m->rgba8, d->rgba8, nv * 4)) != hipSuccess ||
-        (e = up((void **)&m->idx, d->idx, d->count * 4)) != hipSuccess ||
-        (e = up((void **)&m->models, models, (size_t)16 * 4 * d->n_instances)) != hipSuccess) {
+    euchip_status e;
+    if ((e = up((void **)&m->pos, d->pos, nv * d->pos_comps * 4)) || (e = up((void **)&m->nrm, d->nrm, nv * 12)) ||
+        (e = up((void **)&m->col, d->col, nv * 16)) || (e = up((void **)&m->uv, d->uv, nv * 8)) ||
+        (e = up((void **)&m->rgba8, d->rgba8, nv * 4)) || (e = up((void **)&m->idx, d->idx, d->count * 4)) ||
+        (e = up((void **)&m->models, models, (size_t)16 * 4 * d->n_instances))) {
+        std::string msg = ctx->last_error;
         euchip_mesh_destroy(m);
-        return hip_fail(ctx, e, "mesh upload");
+        ctx->last_error = msg;
+        return e;
     }
     *out = m;
     return EUCHIP_OK;
@@ -553,14 +911,11 @@ euchip_status euchip_mesh_create(euchip_ctx *ctx, const euchip_mesh_desc *d, euc
 
 euchip_status euchip_mesh_destroy(euchip_mesh *m) {
     if (!m) return EUCHIP_INVALID_ARG;
-    (void)hipStreamSynchronize(m->ctx->stream);
-    (void)hipFree(m->pos);
-    (void)hipFree(m->nrm);
-    (void)hipFree(m->col);
-    (void)hipFree(m->uv);
-    (void)hipFree(m->rgba8);
-    (void)hipFree(m->idx);
-    (void)hipFree(m->models);
+    (void)hipDeviceSynchronize();  // batches on other streams may still use it
+    for (void *ptr : {(void *)m->pos, (void *)m->nrm, (void *)m->col, (void *)m->uv, (void *)m->rgba8, (void *)m->idx,
+                      (void *)m->models})
+        if (ptr) (void)hipFree(ptr);
+    m->ctx->bytes_in_use -= std::min(m->bytes, m->ctx->bytes_in_use);
     delete m;
     return EUCHIP_OK;
 }
@@ -582,6 +937,166 @@ euchip_status euchip_render_async(euchip_ctx *ctx, const euchip_pipeline_desc *d
                                   const euchip_mesh *mesh, euchip_buffer *pixel, euchip_buffer *depth) {
     if (ctx) (void)hipSetDevice(ctx->device);
     return render_impl(ctx, desc, uniforms, mesh, pixel, depth, false);
+}
+
+euchip_status euchip_batch_create(euchip_ctx *ctx, euchip_batch **out) {
+    if (!ctx || !out) return fail(ctx, EUCHIP_INVALID_ARG, "null argument");
+    (void)hipSetDevice(ctx->device);
+    *out = nullptr;
+    euchip_batch *b = new euchip_batch();
+    b->ctx = ctx;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return fail(ctx, EUCHIP_HIP_ERROR, "batch stream");
+    }
+    *out = b;
+    return EUCHIP_OK;
+}
+
+euchip_status euchip_batch_destroy(euchip_batch *b) {
+    if (!b) return EUCHIP_INVALID_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->stream);
+    free_batch_memory(b);
+    (void)hipEventDestroy(b->done);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return EUCHIP_OK;
+}
+
+euchip_status euchip_batch_set(euchip_batch *b, const euchip_pipeline_desc *desc, const euchip_request *reqs,
+                               uint32_t n) {
+    if (!b || !desc || (n && !reqs)) return fail(b ? b->ctx : nullptr, EUCHIP_INVALID_ARG, "null argument");
+    euchip_ctx *ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    euchip_status s = check_desc(ctx, desc);
+    if (s) return s;
+    HIP_TRY(ctx, hipStreamSynchronize(b->stream));
+    std::vector<Plan> plans(n);
+    std::vector<std::array<float, kUniforms>> uni(n);
+    std::vector<uint32_t> active;
+    for (uint32_t i = 0; i < n; i++) {
+        const euchip_request &r = reqs[i];
+        if (!r.uniforms || !r.mesh) return fail(ctx, EUCHIP_INVALID_ARG, "request %u: null mesh or uniforms", i);
+        memcpy(uni[i].data(), r.uniforms, sizeof(float) * kUniforms);
+        const euchip_buffer *tex = r.texture ? r.texture : desc->texture;
+        const euchip_buffer *shw = r.shadow ? r.shadow : desc->shadow;
+        if ((s = plan_request(ctx, desc, uni[i].data(), r.mesh, r.pixel, r.depth, tex, shw, plans[i])) ||
+            (s = plans[i].status)) {
+            ctx->last_error = "request " + std::to_string(i) + ": " + ctx->last_error;
+            return s;
+        }
+        if (plans[i].render) active.push_back(i);
+    }
+    b->desc = *desc;
+    b->reqs.assign(reqs, reqs + n);
+    b->uniforms = std::move(uni);
+    b->plans = std::move(plans);
+    b->active = std::move(active);
+    for (auto &c : b->caps) c.clear();
+    b->tables_dirty = true;
+    b->launched = false;
+    drop_graph(b);
+    return EUCHIP_OK;
+}
+
+euchip_status euchip_batch_set_tile_width(euchip_batch *b, uint32_t tile_w) {
+    if (!b) return EUCHIP_INVALID_ARG;
+    const uint32_t sx = tile_w == 8 ? 3u : tile_w == 16 ? 4u : tile_w == 32 ? 5u : 0u;
+    if (!sx) return fail(b->ctx, EUCHIP_INVALID_ARG, "tile width %u (8, 16 or 32)", tile_w);
+    if (sx != b->tile_sx) {
+        HIP_TRY(b->ctx, hipStreamSynchronize(b->stream));
+        b->tile_sx = sx;
+        b->tables_dirty = true;
+        drop_graph(b);
+    }
+    return EUCHIP_OK;
+}
+
+euchip_status euchip_batch_launch(euchip_batch *b, int use_graph) {
+    if (!b) return EUCHIP_INVALID_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    return launch_batch(b, use_graph != 0);
+}
+
+euchip_status euchip_batch_wait(euchip_batch *b) {
+    if (!b) return EUCHIP_INVALID_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    return wait_batch(b);
+}
+
+euchip_status euchip_batch_query(euchip_batch *b) {
+    if (!b) return EUCHIP_INVALID_ARG;
+    const hipError_t e = hipEventQuery(b->done);
+    if (e == hipSuccess) return EUCHIP_OK;
+    if (e == hipErrorNotReady) return EUCHIP_NOT_READY;
+    return hip_fail(b->ctx, e, "hipEventQuery");
+}
+
+euchip_status euchip_batch_autotune(euchip_batch *b, uint32_t reps, uint32_t *best_tile_w, float *times_ms) {
+    if (!b) return EUCHIP_INVALID_ARG;
+    euchip_ctx *ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    reps = std::max<uint32_t>(reps, 1u);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(ctx, EUCHIP_HIP_ERROR, "event");
+    }
+    float best = 0.0f;
+    uint32_t best_w = 32;
+    euchip_status s = EUCHIP_OK;
+    for (uint32_t w : {8u, 16u, 32u}) {
+        if ((s = euchip_batch_set_tile_width(b, w))) break;
+        // two calibration launches size the bins for this width
+        for (int i = 0; i < 2 && !s; i++) {
+            if (!(s = launch_batch(b, false))) s = wait_batch(b);
+        }
+        if (s) break;
+        if (b->tables_dirty && (s = build_tables(b))) break;
+        (void)hipEventRecord(e0, b->stream);
+        for (uint32_t i = 0; i < reps && !s; i++) s = enqueue_batch(b);
+        (void)hipEventRecord(e1, b->stream);
+        if (s) break;
+        b->launched = true;
+        if ((s = wait_batch(b))) break;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)reps;
+        if (times_ms) times_ms[(w == 8) ? 0 : (w == 16) ? 1 : 2] = ms;
+        if (w == 8 || ms < best) best = ms, best_w = w;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (s) return s;
+    if (best_tile_w) *best_tile_w = best_w;
+    return euchip_batch_set_tile_width(b, best_w);
+}
+
+void *euchip_batch_stream(euchip_batch *b) { return b ? (void *)b->stream : nullptr; }
+
+size_t euchip_batch_scratch_bytes(const euchip_batch *b) {
+    if (!b) return 0;
+    return b->views.cap * sizeof(View) + b->blk.cap * sizeof(uint2) + b->tile_req.cap * 4 + b->clip.cap * 16 +
+           b->vary.cap * 4 + b->tile_count.cap * 4 + b->bins.cap * 4 + b->ctrbuf.cap * 4 + b->ovf.cap * 8;
+}
+
+size_t euchip_batch_estimate_bytes(const euchip_pipeline_desc *desc, const euchip_request *reqs, uint32_t n) {
+    if (!desc || (n && !reqs)) return 0;
+    size_t total = sizeof(Counters) + 8 * ((size_t)1 << 20);
+    for (uint32_t i = 0; i < n; i++) {
+        const euchip_request &r = reqs[i];
+        if (!r.mesh || !r.uniforms) continue;
+        Plan P;
+        const euchip_buffer *tex = r.texture ? r.texture : desc->texture;
+        const euchip_buffer *shw = r.shadow ? r.shadow : desc->shadow;
+        if (plan_request(r.mesh->ctx, desc, r.uniforms, r.mesh, r.pixel, r.depth, tex, shw, P) || !P.render) continue;
+        total += euchip::request_scratch_estimate(desc, r.mesh, P.v.W, P.v.H);
+    }
+    return total;
 }
 
 }  // extern "C"

@@ -132,60 +132,60 @@ EUCHIP_DEV void sample_bilinear(const void *data, uint32_t w, uint32_t h, float 
 // ---------------------------------------------------------------------------
 
 // World position (model * (p, 1)) or, for CLIP input, the position itself.
-EUCHIP_DEV void vertex_world(const FrameParams &p, uint32_t inst, uint32_t vi, float wpos[4]) {
-    const float *P = p.pos + (size_t)p.pos_comps * vi;
+EUCHIP_DEV void vertex_world(const FrameParams &p, const View &v, uint32_t inst, uint32_t vi, float wpos[4]) {
+    const float *P = v.pos + (size_t)v.pos_comps * vi;
     if (p.xform == 0) {
         wpos[0] = P[0], wpos[1] = P[1], wpos[2] = P[2];
-        wpos[3] = p.pos_comps == 4 ? P[3] : 1.0f;
+        wpos[3] = v.pos_comps == 4 ? P[3] : 1.0f;
     } else {
         const float p4[4] = {P[0], P[1], P[2], 1.0f};
-        mat4_vec4(p.models + 16u * inst, p4, wpos);
+        mat4_vec4(v.models + 16u * inst, p4, wpos);
     }
 }
 
 // Homogeneous clip position returned by vertex().
-EUCHIP_DEV void vertex_clip(const FrameParams &p, uint32_t inst, uint32_t vi, float clip[4]) {
+EUCHIP_DEV void vertex_clip(const FrameParams &p, const View &v, uint32_t inst, uint32_t vi, float clip[4]) {
     float w[4];
-    vertex_world(p, inst, vi, w);
+    vertex_world(p, v, inst, vi, w);
     if (p.xform == 0) {
         clip[0] = w[0], clip[1] = w[1], clip[2] = w[2], clip[3] = w[3];
     } else {
-        mat4_vec4(p.u + U_VP, w, clip);
+        mat4_vec4(v.u + U_VP, w, clip);
     }
 }
 
 // VertexData returned by vertex().
 template <uint32_t SH>
-EUCHIP_DEV void vertex_data(const FrameParams &p, uint32_t inst, uint32_t vi, float *vd) {
+EUCHIP_DEV void vertex_data(const FrameParams &p, const View &v, uint32_t inst, uint32_t vi, float *vd) {
     if constexpr (SH == FLAT) {
-        vd[0] = __uint_as_float(p.rgba8[vi]);
+        vd[0] = __uint_as_float(v.rgba8[vi]);
     } else if constexpr (SH == COLOR) {
-        const float *c = p.col + 4u * vi;
+        const float *c = v.col + 4u * vi;
         vd[0] = c[0], vd[1] = c[1], vd[2] = c[2], vd[3] = c[3];
     } else if constexpr (SH == TEXTURED) {
-        vd[0] = p.uv[2u * vi], vd[1] = p.uv[2u * vi + 1];
+        vd[0] = v.uv[2u * vi], vd[1] = v.uv[2u * vi + 1];
     } else if constexpr (SH == DEPTH_ONLY) {
         (void)vd;
     } else {
         float wpos[4], wn[4];
-        vertex_world(p, inst, vi, wpos);
-        const float *N = p.nrm + 3u * vi;
+        vertex_world(p, v, inst, vi, wpos);
+        const float *N = v.nrm + 3u * vi;
         float n4[4] = {N[0], N[1], N[2], 0.0f};
         if constexpr (SH == PHONG_SHADOW) n4[0] = -N[0], n4[1] = -N[1], n4[2] = -N[2];
         if (p.xform == 0) wn[0] = n4[0], wn[1] = n4[1], wn[2] = n4[2], wn[3] = n4[3];
-        else mat4_vec4(p.models + 16u * inst, n4, wn);
+        else mat4_vec4(v.models + 16u * inst, n4, wn);
         if constexpr (SH == GOURAUD) {
-            lighting2(p.u, wpos, wn, vd);
+            lighting2(v.u, wpos, wn, vd);
         } else {
             vd[0] = wpos[0], vd[1] = wpos[1], vd[2] = wpos[2];
             vd[3] = wn[0], vd[4] = wn[1], vd[5] = wn[2];
             if constexpr (SH == PHONG_SHADOW || SH == TEX_SHADOW) {
                 const float w1[4] = {wpos[0], wpos[1], wpos[2], 1.0f};
                 float l4[4];
-                mat4_vec4(p.u + U_LVP, w1, l4);
+                mat4_vec4(v.u + U_LVP, w1, l4);
                 int o = 6;
                 if constexpr (SH == TEX_SHADOW) {
-                    vd[6] = p.uv[2u * vi], vd[7] = p.uv[2u * vi + 1];
+                    vd[6] = v.uv[2u * vi], vd[7] = v.uv[2u * vi + 1];
                     o = 8;
                 }
                 vd[o + 0] = l4[0] / l4[3];
@@ -197,19 +197,19 @@ EUCHIP_DEV void vertex_data(const FrameParams &p, uint32_t inst, uint32_t vi, fl
 }
 
 // Shadow-map lookup of examples/teapot.rs:123-128.
-EUCHIP_DEV bool in_light(const FrameParams &p, const float lvp[3]) {
+EUCHIP_DEV bool in_light(const View &v, const float lvp[3]) {
     float sx = (lvp[0] * 1.0f) * 0.5f + 0.5f;
     float sy = (lvp[1] * -1.0f) * 0.5f + 0.5f;
     float ld;
-    sample_bilinear<1>(p.shadow, p.sh_w, p.sh_h, sx, sy, true, &ld);
-    ld = ld + p.u[U_K + 3];
+    sample_bilinear<1>(v.shadow, v.sh_w, v.sh_h, sx, sy, true, &ld);
+    ld = ld + v.u[U_K + 3];
     return lvp[2] < ld;
 }
 
 // Pipeline::fragment of the fixed shader set.  out: Rgba<f32> (FLAT: colour bits in out[0]).
 template <uint32_t SH>
-EUCHIP_DEV void fragment_shader(const FrameParams &p, const float *vd, float out[4]) {
-    const float *u = p.u;
+EUCHIP_DEV void fragment_shader(const View &v, const float *vd, float out[4]) {
+    const float *u = v.u;
     out[0] = out[1] = out[2] = out[3] = 0.0f;
     if constexpr (SH == FLAT) {
         out[0] = vd[0];
@@ -218,7 +218,7 @@ EUCHIP_DEV void fragment_shader(const FrameParams &p, const float *vd, float out
     } else if constexpr (SH == PHONG) {
         lighting2(u, vd, vd + 3, out);
     } else if constexpr (SH == TEXTURED) {
-        sample_bilinear<4>(p.tex, p.tex_w, p.tex_h, vd[0], vd[1], false, out);
+        sample_bilinear<4>(v.tex, v.tex_w, v.tex_h, vd[0], vd[1], false, out);
     } else if constexpr (SH == PHONG_SHADOW) {
         float wn[3], cd[3], ld[3], t3[3];
         normalize3(vd + 3, wn);
@@ -232,14 +232,14 @@ EUCHIP_DEV void fragment_shader(const FrameParams &p, const float *vd, float out
         float r[3] = {nl[0] - wn[0] * dn, nl[1] - wn[1] * dn, nl[2] - wn[2] * dn};
         float ncd[3] = {-cd[0], -cd[1], -cd[2]};
         float specular = pow30(rmax(dot3s(r, ncd), 0.0f)) * u[U_K + 2];
-        float light = u[U_K + 0] + (in_light(p, vd + 6) ? diffuse + specular : 0.0f);
+        float light = u[U_K + 0] + (in_light(v, vd + 6) ? diffuse + specular : 0.0f);
         for (int c = 0; c < 4; c++) out[c] = u[U_SURF + c] * light;
     } else if constexpr (SH == TEX_SHADOW) {
         float n[3], texc[4];
         normalize3(vd + 3, n);
         float ndl = rmax(dot3s(n, u + U_L0), 0.0f);
-        float light = u[U_K + 0] + (in_light(p, vd + 8) ? ndl * u[U_K + 1] : 0.0f);
-        sample_bilinear<4>(p.tex, p.tex_w, p.tex_h, vd[6], vd[7], true, texc);
+        float light = u[U_K + 0] + (in_light(v, vd + 8) ? ndl * u[U_K + 1] : 0.0f);
+        sample_bilinear<4>(v.tex, v.tex_w, v.tex_h, vd[6], vd[7], true, texc);
         float s = light * (1.0f / 255.0f);
         out[0] = texc[0] * s;
         out[1] = texc[1] * s;

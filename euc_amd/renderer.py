@@ -16,7 +16,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import EuchipError, MeshDescC, PipelineDescC, KernelStatC, FORMAT_F32, FORMAT_RGBA8
+from ._lib import EuchipError, MeshDescC, PipelineDescC, KernelStatC, RequestC, FORMAT_F32, FORMAT_RGBA8
 from .pipeline import (AaMode, BandPolicy, CoordinateMode, CullMode, DepthMode, PipelineDesc, PixelMode,
                        Shader, Xform)
 from .scenes import Mesh, N_UNIFORMS
@@ -124,6 +124,113 @@ class GpuMesh:
             pass
 
 
+def desc_c(desc: PipelineDesc, texture=None, shadow=None, clear_color=None, clear_depth=None) -> PipelineDescC:
+    """The C pipeline descriptor of `desc` plus textures and in-frame clears."""
+    return PipelineDescC(
+        shader=int(desc.shader), xform=int(desc.xform), pixel_write=int(desc.pixel_write),
+        depth_test=int(desc.depth_test), depth_write=int(desc.depth_write), y_up=int(desc.y_up),
+        z_clip=int(desc.z_clip), z_min=float(desc.z_min), z_max=float(desc.z_max), cull=int(desc.cull),
+        aa_msaa=int(desc.aa_msaa), aa_level=max(0, int(desc.aa_level)), band_policy=int(desc.band_policy),
+        threads=int(desc.threads),
+        clear_color_enable=int(clear_color is not None),
+        clear_color=0 if clear_color is None else int(clear_color) & 0xFFFFFFFF,
+        clear_depth_enable=int(clear_depth is not None),
+        clear_depth=0.0 if clear_depth is None else float(clear_depth),
+        texture=texture.handle if texture else None, shadow=shadow.handle if shadow else None)
+
+
+class Request:
+    """One render() of a batch: a mesh into its own targets with its own uniforms."""
+
+    def __init__(self, mesh: "GpuMesh", pixel: Optional["Buffer2d"], depth: Optional["Buffer2d"],
+                 uniforms: np.ndarray, texture: Optional["Buffer2d"] = None, shadow: Optional["Buffer2d"] = None):
+        self.mesh, self.pixel, self.depth = mesh, pixel, depth
+        self.uniforms = np.ascontiguousarray(uniforms, dtype=np.float32)
+        assert self.uniforms.size == N_UNIFORMS
+        self.texture, self.shadow = texture, shadow
+
+
+def _requests_c(reqs):
+    arr = (RequestC * max(len(reqs), 1))()
+    for i, r in enumerate(reqs):
+        arr[i] = RequestC(mesh=r.mesh.handle, pixel=r.pixel.handle if r.pixel else None,
+                          depth=r.depth.handle if r.depth else None, uniforms=r.uniforms.ctypes.data,
+                          texture=r.texture.handle if r.texture else None,
+                          shadow=r.shadow.handle if r.shadow else None)
+    return arr
+
+
+def estimate_bytes(desc: PipelineDesc, reqs, texture=None, shadow=None) -> int:
+    """Device bytes a batch of `reqs` adds as scratch (the admission estimate)."""
+    d = desc_c(desc, texture, shadow)
+    return int(_lib.lib().euchip_batch_estimate_bytes(C.byref(d), _requests_c(reqs), len(reqs)))
+
+
+class Batch:
+    """Many render() calls of one pipeline in one setup + one raster launch, on
+    the batch's own stream; each request's targets end up as euchip_render of
+    that request alone would leave them."""
+
+    def __init__(self, ctx: "Context"):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(ctx.handle, _lib.lib().euchip_batch_create(ctx.handle, C.byref(h)))
+        self.handle = h.value
+        self.requests = []
+        ctx._children.add(self)
+
+    def set(self, desc: PipelineDesc, reqs, texture=None, shadow=None, clear_color=None, clear_depth=None):
+        d = desc_c(desc, texture, shadow, clear_color, clear_depth)
+        arr = _requests_c(reqs)
+        self.requests = list(reqs)  # keeps uniforms and targets alive
+        _check(self.ctx.handle, _lib.lib().euchip_batch_set(self.handle, C.byref(d), arr, len(reqs)))
+        return self
+
+    def set_tile_width(self, w: int):
+        _check(self.ctx.handle, _lib.lib().euchip_batch_set_tile_width(self.handle, int(w)))
+
+    def autotune(self, reps: int = 3):
+        """Time tile widths 8, 16, 32; keep the fastest.  Returns (width, {width: ms})."""
+        best = C.c_uint32()
+        times = (C.c_float * 3)()
+        _check(self.ctx.handle, _lib.lib().euchip_batch_autotune(self.handle, int(reps), C.byref(best), times))
+        return int(best.value), {8: times[0], 16: times[1], 32: times[2]}
+
+    def launch(self, graph: bool = False):
+        _check(self.ctx.handle, _lib.lib().euchip_batch_launch(self.handle, int(graph)))
+
+    def wait(self):
+        _check(self.ctx.handle, _lib.lib().euchip_batch_wait(self.handle))
+
+    def done(self) -> bool:
+        s = _lib.lib().euchip_batch_query(self.handle)
+        if s == 8:
+            return False
+        _check(self.ctx.handle, s)
+        return True
+
+    def run(self, graph: bool = False):
+        self.launch(graph)
+        self.wait()
+
+    def stream(self) -> int:
+        return _lib.lib().euchip_batch_stream(self.handle)
+
+    def scratch_bytes(self) -> int:
+        return int(_lib.lib().euchip_batch_scratch_bytes(self.handle))
+
+    def close(self):
+        if self.handle and self.ctx.handle:
+            _lib.lib().euchip_batch_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One device context (one HIP stream)."""
 
@@ -157,17 +264,7 @@ class Context:
                clear_depth: Optional[float] = None, sync: bool = True):
         u = np.ascontiguousarray(uniforms, dtype=np.float32)
         assert u.size == N_UNIFORMS
-        d = PipelineDescC(
-            shader=int(desc.shader), xform=int(desc.xform), pixel_write=int(desc.pixel_write),
-            depth_test=int(desc.depth_test), depth_write=int(desc.depth_write), y_up=int(desc.y_up),
-            z_clip=int(desc.z_clip), z_min=float(desc.z_min), z_max=float(desc.z_max), cull=int(desc.cull),
-            aa_msaa=int(desc.aa_msaa), aa_level=max(0, int(desc.aa_level)), band_policy=int(desc.band_policy),
-            threads=int(desc.threads),
-            clear_color_enable=int(clear_color is not None),
-            clear_color=0 if clear_color is None else int(clear_color) & 0xFFFFFFFF,
-            clear_depth_enable=int(clear_depth is not None),
-            clear_depth=0.0 if clear_depth is None else float(clear_depth),
-            texture=texture.handle if texture else None, shadow=shadow.handle if shadow else None)
+        d = desc_c(desc, texture, shadow, clear_color, clear_depth)
         L = _lib.lib()
         f = L.euchip_render if sync else L.euchip_render_async
         _check(self.handle, f(self.handle, C.byref(d), u.ctypes.data, mesh.handle,
@@ -175,6 +272,17 @@ class Context:
 
     def synchronize(self):
         _check(self.handle, _lib.lib().euchip_synchronize(self.handle))
+
+    def bytes_in_use(self) -> int:
+        """Device bytes held through this context (targets, meshes, scratch)."""
+        return int(_lib.lib().euchip_ctx_bytes_in_use(self.handle))
+
+    def set_budget(self, nbytes: int):
+        """Device-memory budget (0 = none): allocations past it fail with OVER_BUDGET."""
+        _check(self.handle, _lib.lib().euchip_ctx_set_budget(self.handle, int(nbytes)))
+
+    def batch(self) -> "Batch":
+        return Batch(self)
 
     def stream(self) -> int:
         return _lib.lib().euchip_ctx_stream(self.handle)
@@ -195,7 +303,11 @@ class Context:
     def close(self):
         """Destroy every buffer and mesh of this context, then the context."""
         if self.handle:
-            for c in list(self._children):
+            kids = list(self._children)
+            for c in kids:  # batches and servers first: they use the buffers and meshes
+                if not isinstance(c, (Buffer2d, GpuMesh)):
+                    c.close()
+            for c in kids:
                 c.close()
             _lib.lib().euchip_ctx_destroy(self.handle)
             self.handle = None

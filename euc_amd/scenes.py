@@ -289,6 +289,16 @@ def value_noise(seed, size, cells):
     return (a * (1 - fy) + b * fy) * (1 - fx) + (c * (1 - fy) + d * fy) * fx
 
 
+def noise_texture(seed, size):
+    """size x size RGBA8 (packed u32) texture from two octaves of value noise."""
+    tex_n = value_noise(seed + 99, size, 32)
+    tex_m = value_noise(seed + 98, size, 7)
+    rr = (40 + 150 * tex_m + 60 * tex_n).clip(0, 255).astype(np.uint32)
+    gg = (90 + 120 * tex_n).clip(0, 255).astype(np.uint32)
+    bb = (30 + 80 * (1 - tex_m)).clip(0, 255).astype(np.uint32)
+    return (rr | (gg << 8) | (bb << 16) | (np.uint32(255) << 24)).astype(np.uint32)
+
+
 def heightfield_scene(seed=0x5EED0000 + 3, width=1920, height=1080, grid=317, tex_size=1024,
                       shadow_size=2048) -> Scene:
     """Seeded value-noise heightfield (grid x grid vertices), textured, lit, shadowed."""
@@ -308,12 +318,7 @@ def heightfield_scene(seed=0x5EED0000 + 3, width=1920, height=1080, grid=317, te
     c = (i + 1) * grid + j + 1
     d = (i + 1) * grid + j
     tris = np.stack([a, b, c, a, c, d], -1).reshape(-1).astype(np.uint32)
-    tex_n = value_noise(seed + 99, tex_size, 32)
-    tex_m = value_noise(seed + 98, tex_size, 7)
-    rr = (40 + 150 * tex_m + 60 * tex_n).clip(0, 255).astype(np.uint32)
-    gg = (90 + 120 * tex_n).clip(0, 255).astype(np.uint32)
-    bb = (30 + 80 * (1 - tex_m)).clip(0, 255).astype(np.uint32)
-    tex = (rr | (gg << 8) | (bb << 16) | (np.uint32(255) << 24)).astype(np.uint32)
+    tex = noise_texture(seed, tex_size)
     cam = np.array([0.0, 3.5, -6.5])
     view = look_at_lh(cam, (0.0, 0.0, 0.5), (0.0, 1.0, 0.0))
     vp = perspective_fov_lh_zo(1.1, width, height, 0.05, 100.0) @ view
@@ -359,3 +364,50 @@ def icon_request(seed: int, min_tris=200, max_tris=20000, min_side=32, max_side=
     u = uniforms(view_proj=f32(vp), lights=[((-0.3, -0.6, -0.7), 1.0), ((0.7, 0.3, -0.4), 0.5)])
     mesh = Mesh(pos=pos, nrm=nrm, idx=idx, count=int(idx.size), n_instances=1, models=model)
     return Scene(f"icon{seed:x}", mesh, u, side, side, 0, 1.0)
+
+
+# --------------------------------------------------------------------------- #
+# Serving workload: resident textured meshes, requests with their own camera
+# --------------------------------------------------------------------------- #
+
+def textured_mesh(seed: int, n_tris: int) -> Mesh:
+    """A sphere or torus of about n_tris triangles with UVs from its parametrisation."""
+    r = rand_f32(seed, 2).astype(np.float64)
+    if r[0] < 0.5:
+        nu = max(3, int(round(np.sqrt(n_tris))))
+        nv = max(2, int(round(n_tris / (2.0 * nu))))
+        pos, nrm, idx = uv_sphere(nu, nv)
+        U, V = np.meshgrid(np.arange(nu) / nu, np.linspace(0, 1, nv + 1), indexing="ij")
+    else:
+        nu = max(3, int(round(np.sqrt(n_tris) * 1.4)))
+        nv = max(3, int(round(n_tris / (2.0 * nu))))
+        pos, nrm, idx = torus(1.0, 0.4, nu, nv)
+        pos = f32(pos.astype(np.float64) / 1.4)
+        U, V = np.meshgrid(np.arange(nu) / nu, np.arange(nv) / nv, indexing="ij")
+    uv = f32(np.stack([U * 4.0, V * 2.0], -1).reshape(-1, 2))
+    return Mesh(pos=pos, nrm=nrm, uv=uv, idx=idx, count=int(idx.size), n_instances=1,
+                models=np.eye(4, dtype=np.float32).reshape(1, 16))
+
+
+def serving_assets(seed=0x5EED0000 + 5, n_meshes=12, n_textures=6, min_tris=1000, max_tris=500000,
+                   tex_sizes=(256, 512, 1024, 2048)):
+    """Resident data of the serving workload: meshes with log-uniform triangle
+    counts (the first and last at the bounds) and value-noise textures."""
+    r = rand_f32(seed, n_meshes).astype(np.float64)
+    r[0], r[-1] = 0.0, 1.0
+    counts = np.exp(np.log(min_tris) + r * (np.log(max_tris) - np.log(min_tris))).astype(int)
+    meshes = [textured_mesh(seed + 1 + i, int(n)) for i, n in enumerate(counts)]
+    textures = [noise_texture(seed + 100 + i, tex_sizes[i % len(tex_sizes)]) for i in range(n_textures)]
+    return meshes, textures
+
+
+def serving_request(seed: int, n_meshes: int, n_textures: int, min_side=64, max_side=1024):
+    """(mesh index, texture index, uniforms, side): a seeded camera on a sphere
+    of radius 3.2 around the mesh, a square target of log-uniform side."""
+    r = rand_f32(seed, 6).astype(np.float64)
+    side = int(np.exp(np.log(min_side) + r[0] * (np.log(max_side) - np.log(min_side))))
+    side = max(min_side, min(max_side, side))
+    th, ph = r[1] * 2 * np.pi, (r[2] - 0.5) * 2.6
+    eye = 3.2 * np.array([np.cos(ph) * np.cos(th), np.sin(ph), np.cos(ph) * np.sin(th)])
+    vp = perspective_fov_lh_zo(0.9, side, side, 0.1, 20.0) @ look_at_lh(eye, (0.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+    return int(r[3] * n_meshes) % n_meshes, int(r[4] * n_textures) % n_textures, uniforms(view_proj=f32(vp)), side

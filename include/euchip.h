@@ -171,6 +171,105 @@ euchip_status euchip_render(euchip_ctx *ctx, const euchip_pipeline_desc *desc, c
 euchip_status euchip_render_async(euchip_ctx *ctx, const euchip_pipeline_desc *desc, const float *uniforms,
                                   const euchip_mesh *mesh, euchip_buffer *pixel, euchip_buffer *depth);
 
+/* ---- device memory budget ---------------------------------------------------------- */
+/* Bytes of device memory the context holds (buffers, meshes, per-frame and batch
+ * scratch).  With a budget set (0 = none), an allocation that would take the total
+ * past it fails with EUCHIP_OVER_BUDGET and allocates nothing. */
+size_t euchip_ctx_bytes_in_use(const euchip_ctx *ctx);
+euchip_status euchip_ctx_set_budget(euchip_ctx *ctx, size_t bytes);
+
+/* ---- batched rendering -------------------------------------------------------------
+ * Many render() calls of one pipeline in one setup launch and one raster launch:
+ * each request has its own mesh, targets and uniform block, and its results are
+ * those of euchip_render() on that request alone (same targets, same uniforms).
+ * A batch owns its stream (default priority) and its scratch memory, so batches
+ * can be in flight together and with the context's own stream.  No counterpart
+ * in euc: it is the serving form of Pipeline::render (src/pipeline.rs:248-300)
+ * applied to many independent targets. */
+typedef struct euchip_batch euchip_batch;
+
+typedef struct euchip_request {
+    const euchip_mesh *mesh;
+    euchip_buffer *pixel;          /* RGBA8 target or NULL (Empty) */
+    euchip_buffer *depth;          /* f32 target or NULL (Empty) */
+    const float *uniforms;         /* EUCHIP_UNIFORM_FLOATS host floats, copied by batch_set */
+    const euchip_buffer *texture;  /* NULL: the pipeline desc's texture */
+    const euchip_buffer *shadow;   /* NULL: the pipeline desc's shadow map */
+} euchip_request;
+
+euchip_status euchip_batch_create(euchip_ctx *ctx, euchip_batch **out);
+euchip_status euchip_batch_destroy(euchip_batch *b);
+/* Replace the requests (desc and requests are copied; scratch grows, never
+ * shrinks).  Fails with the first request's error (EUCHIP_SIZE_MISMATCH,
+ * EUCHIP_BAND_ZERO, EUCHIP_INVALID_ARG) naming its index in last_error. */
+euchip_status euchip_batch_set(euchip_batch *b, const euchip_pipeline_desc *desc, const euchip_request *reqs,
+                               uint32_t n);
+/* Raster tile shape: tile_w x (256 / tile_w) pixels, tile_w 8, 16 or 32. */
+euchip_status euchip_batch_set_tile_width(euchip_batch *b, uint32_t tile_w);
+/* Launch the batch `reps` times at each tile width, keep the fastest.
+ * best_tile_w (may be NULL) gets it; times_ms (may be NULL) gets the mean launch
+ * time of widths 8, 16, 32. */
+euchip_status euchip_batch_autotune(euchip_batch *b, uint32_t reps, uint32_t *best_tile_w, float *times_ms);
+/* Enqueue the batch on its stream.  use_graph: replay a hipGraph of the launch
+ * (captured on first use, recaptured after set / set_tile_width / bin growth). */
+euchip_status euchip_batch_launch(euchip_batch *b, int use_graph);
+/* Wait for the last launch.  A launch whose bins overflowed their storage is
+ * re-run with larger bins before this returns, so on EUCHIP_OK every target holds
+ * its result. */
+euchip_status euchip_batch_wait(euchip_batch *b);
+/* EUCHIP_OK when the last launch has finished, EUCHIP_NOT_READY otherwise. */
+euchip_status euchip_batch_query(euchip_batch *b);
+void *euchip_batch_stream(euchip_batch *b);
+/* Device bytes of the batch's scratch (vertices, bins, tables). */
+size_t euchip_batch_scratch_bytes(const euchip_batch *b);
+/* Device bytes a batch of these requests would add to an empty batch's scratch at
+ * tile width 32: the admission estimate of a serving loop. */
+size_t euchip_batch_estimate_bytes(const euchip_pipeline_desc *desc, const euchip_request *reqs, uint32_t n);
+
+/* ---- serving loop ------------------------------------------------------------------
+ * Continuous batching over resident meshes and textures: submitted requests wait
+ * in a FIFO queue; each poll retires finished batches and admits queued requests
+ * into free batch slots while the context's device memory (targets, scratch, the
+ * resident data) stays within the budget, then launches them.  Slots run on their
+ * own streams, so a batch admitted while another is on the GPU overlaps with it.
+ * Each request's targets hold exactly what euchip_render of that request alone
+ * would leave.  No counterpart in euc (a serving layer over Pipeline::render). */
+typedef struct euchip_server euchip_server;
+
+typedef struct euchip_serve_request {
+    const euchip_mesh *mesh;       /* resident mesh */
+    const euchip_buffer *texture;  /* NULL: the pipeline desc's texture */
+    const euchip_buffer *shadow;   /* NULL: the pipeline desc's shadow map */
+    const float *uniforms;         /* EUCHIP_UNIFORM_FLOATS host floats, copied */
+    uint32_t width, height;        /* the server allocates an RGBA8 and an f32 target of this size */
+} euchip_serve_request;
+
+typedef struct euchip_completion {
+    uint64_t ticket;
+    euchip_status status;          /* EUCHIP_OVER_BUDGET: the request alone does not fit the budget */
+    euchip_buffer *pixel;          /* valid until euchip_server_release(ticket) */
+    euchip_buffer *depth;
+} euchip_completion;
+
+typedef struct euchip_server_stats {
+    uint64_t submitted, admitted, completed, rejected, batches;
+    uint32_t queued, in_flight;
+    size_t bytes_in_use, peak_bytes, budget;
+} euchip_server_stats;
+
+/* budget_bytes: device bytes the context may hold (0 = no limit; it becomes the
+ * context's budget); max_batch: requests per batch; slots: batches in flight. */
+euchip_status euchip_server_create(euchip_ctx *ctx, const euchip_pipeline_desc *desc, size_t budget_bytes,
+                                   uint32_t max_batch, uint32_t slots, euchip_server **out);
+euchip_status euchip_server_destroy(euchip_server *srv);
+euchip_status euchip_server_submit(euchip_server *srv, const euchip_serve_request *req, uint64_t *ticket);
+/* Retire finished batches, admit and launch queued requests; copies up to max
+ * completions (oldest first) to out and their number to *n_out. */
+euchip_status euchip_server_poll(euchip_server *srv, euchip_completion *out, uint32_t max, uint32_t *n_out);
+/* Hand a completed request's targets back to the server's pool. */
+euchip_status euchip_server_release(euchip_server *srv, uint64_t ticket);
+euchip_status euchip_server_get_stats(const euchip_server *srv, euchip_server_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

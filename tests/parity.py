@@ -80,3 +80,9 @@ def channel_mismatch(r):
     b = r["g_color"].view(np.uint8).astype(np.int16)
     d = np.abs(a - b)
     return int(d.max()) if d.size else 0, float((r["o_color"] != r["g_color"]).mean())
+
+
+def assert_same(g_color, g_depth, r_color, r_depth, what=""):
+    """Bit-exact comparison of colour (uint32) and depth (f32 bits) arrays."""
+    assert_bit_exact({"o_rc": 0, "g_rc": 0, "o_color": r_color, "g_color": g_color,
+                      "o_depth": r_depth, "g_depth": g_depth}, what)
