@@ -36,7 +36,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--shader", default="phong", choices=["phong", "gouraud"])
-    ap.add_argument("--workload", default="torus", choices=["torus", "icons"])
+    ap.add_argument("--workload", default="torus", choices=["torus", "icons", "serving"])
+    ap.add_argument("--rate", type=float, default=4000.0, help="Poisson arrival rate of the serving workload (req/s)")
+    ap.add_argument("--requests", type=int, default=3000, help="requests of the serving workload")
+    ap.add_argument("--budget-gib", type=float, default=16.0, help="device-memory budget of the serving workload")
     ap.add_argument("--icons", type=int, default=4096, help="requests per batch of the icons workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
@@ -237,6 +240,144 @@ def bench_icons(args):
     ctx.close()
 
 
+SERVE_SEED = 0x5EED0000 + 5
+
+
+def bench_serving(args):
+    """Serving loop: `--requests` textured renders arriving as a Poisson stream of
+    `--rate` req/s; resident meshes of 1k-500k triangles and 256^2-2048^2 RGBA8
+    textures; square targets 64..1024 px (log-uniform), allocated per request
+    from a pool; requests admitted into 2 batch slots of up to 64 under a
+    `--budget-gib` device budget.  Reported: frames/s and p50/p99 latency
+    (completion seen by the host minus arrival), next to the plain path (one
+    render() per request as it arrives) and a saturation run (all requests at
+    once) of both.  The timed region is the whole stream; steps = requests."""
+    from euc_amd import scenes as S
+    from euc_amd.pipeline import BandPolicy, CullMode, DepthMode, PipelineDesc, Shader, Xform
+    from euc_amd.renderer import Context
+    from euc_amd.serving import ServeItem, Server, poisson_arrivals, run_poisson
+
+    desc = PipelineDesc.make(Shader.TEXTURED, Xform.MVP, depth_mode=DepthMode.LESS_WRITE, cull=CullMode.BACK,
+                             band_policy=BandPolicy.SEQ)
+    meshes, texs = S.serving_assets(SERVE_SEED)
+    ctx = Context(0)
+    gm = [ctx.mesh(m) for m in meshes]
+    gt = [ctx.texture(t) for t in texs]
+    resident = ctx.bytes_in_use()
+    n = args.requests
+    items, hosts = [], []
+    for i in range(n):
+        mi, ti, u, side = S.serving_request(SERVE_SEED * 7919 + i, len(gm), len(gt))
+        items.append(ServeItem(gm[mi], u, side, side, gt[ti]))
+        hosts.append((mi, ti, u, side))
+    budget = int(args.budget_gib * (1 << 30))
+
+    srv = Server(ctx, desc, budget_bytes=budget, max_batch=64, slots=2, clear_color=0, clear_depth=1.0)
+
+    def served(rate, seed, keep=None):
+        return run_poisson(srv, items, rate_hz=rate, seed=seed, keep=keep)
+
+    def plain(rate, seed):
+        """One render() per request as it arrives (targets from a per-size pool)."""
+        arrivals = poisson_arrivals(n, rate, seed)
+        pool = {}
+        lat = np.empty(n)
+        t0 = time.perf_counter()
+        for i, it in enumerate(items):
+            while time.perf_counter() - t0 < arrivals[i]:
+                pass
+            tg = pool.get(it.width)
+            if tg is None:
+                tg = pool[it.width] = (ctx.color_target(it.width, it.height), ctx.depth_target(it.width, it.height))
+            ctx.render(desc, it.uniforms, it.mesh, tg[0], tg[1], texture=it.texture, clear_color=0, clear_depth=1.0)
+            lat[i] = time.perf_counter() - t0 - arrivals[i]
+        el = time.perf_counter() - t0 - arrivals[0]
+        for c, d in pool.values():
+            c.close()
+            d.close()
+        return lat, el
+
+    # warm-up: a short stream through both paths sizes the scratch and the pools
+    served(args.rate, 99)
+    plain(args.rate * 4, 98)
+    sample = list(range(0, n, max(1, n // 48)))
+    res = served(args.rate, 1, keep=lambda i: i in sample)
+    ok = all(s == 0 for s in res.statuses)
+    frames_per_s = n / res.elapsed_s
+    lat_ms = res.latency_s * 1e3
+    sat = served(1e9, 2)
+    srv.close()
+    ctx.set_budget(0)
+    sp_lat, sp_el = plain(1e9, 2)
+    p_lat, p_el = plain(args.rate, 1)
+    # results in the mix == each request rendered alone
+    parity = ok
+    for i in sample:
+        it = items[i]
+        c = ctx.color_target(it.width, it.height)
+        d = ctx.depth_target(it.width, it.height)
+        ctx.render(desc, it.uniforms, it.mesh, c, d, texture=it.texture, clear_color=0, clear_depth=1.0)
+        parity = parity and np.array_equal(c.download(), res.outputs[i][0]) and \
+            np.array_equal(d.download().view(np.uint32), res.outputs[i][1].view(np.uint32))
+        c.close()
+        d.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import concurrent.futures as cf
+        import threading
+        from oracle import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        t_end = time.perf_counter() + args.cpu_seconds
+        cnt = [0]
+        lock = threading.Lock()
+
+        def worker(k):
+            i, m = k, 0
+            while time.perf_counter() < t_end:
+                mi, ti, u, side = hosts[i % n]
+                pix = np.zeros((side, side), np.uint32)
+                dep = np.ones((side, side), np.float32)
+                rc, _ = O.render(desc, u, meshes[mi], pix, dep, texture=texs[ti])
+                assert rc == 0
+                m += 1
+                i += cores
+            with lock:
+                cnt[0] += m
+
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(cores) as ex:
+            list(ex.map(worker, range(cores)))
+        el = time.perf_counter() - t0
+        cpu = {"value": cnt[0] / el, "unit": "frames/s", "cores": cores, "kind": "port",
+               "sample": f"{cnt[0]} requests of this stream (in order, one host thread each, {cores} threads) "
+                         f"in {el:.1f} s, oracle render_seq"}
+
+    st = res.stats
+    line = {
+        "metric": "served frames/s at a fixed Poisson rate, p50/p99 latency",
+        "value": frames_per_s, "unit": "frames/s",
+        "n_gpus": 1, "steps": n, "warmup": 1, "ms_per_step": res.elapsed_s * 1e3 / n,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n} textured render requests, Poisson {args.rate:.0f} req/s, targets 64-1024 px "
+                               f"square RGBA8+f32, {len(meshes)} resident meshes 1k-500k tris, {len(texs)} textures "
+                               f"256-2048 px, budget {args.budget_gib:g} GiB, 2 slots x 64 requests",
+                   "rate_hz": args.rate, "requests": n, "budget_bytes": budget, "resident_bytes": resident},
+        "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+        "server": {k: st[k] for k in ("batches", "rejected", "peak_bytes")},
+        "plain": {"frames_per_s": n / p_el, "p50_ms": float(np.percentile(p_lat * 1e3, 50)),
+                  "p99_ms": float(np.percentile(p_lat * 1e3, 99))},
+        "saturation": {"served_frames_per_s": n / sat.elapsed_s,
+                       "served_p50_ms": float(np.percentile(sat.latency_s * 1e3, 50)),
+                       "served_p99_ms": float(np.percentile(sat.latency_s * 1e3, 99)),
+                       "plain_frames_per_s": n / sp_el, "batches": sat.stats["batches"]},
+        "results_equal_alone": parity,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -248,6 +389,9 @@ def main():
         return
     if args.workload == "icons":
         bench_icons(args)
+        return
+    if args.workload == "serving":
+        bench_serving(args)
         return
 
     from euc_amd import scenes as S

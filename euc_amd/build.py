@@ -27,6 +27,10 @@ HIP_FLAGS = [
 ]
 
 
+CPP_CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "pipeline_check.cpp")
+CPP_CHECK = os.path.join(ROOT, "tests", "cpp", "pipeline_check")
+
+
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *SOURCES, "-I", os.path.join(ROOT, "include")]
@@ -34,7 +38,20 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_cpp_check(verbose)
     return LIB
+
+
+def build_cpp_check(verbose: bool = False) -> str:
+    """The C++ host-mirror program the GPU tests run (g++ against libeuchip.so)."""
+    tmp = CPP_CHECK + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), CPP_CHECK_SRC, "-o", tmp,
+           "-L", HERE, "-leuchip", "-Wl,-rpath,$ORIGIN/../../euc_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, CPP_CHECK)
+    return CPP_CHECK
 
 
 if __name__ == "__main__":

@@ -24,18 +24,28 @@ constexpr __host__ __device__ uint32_t vd_count(uint32_t shader) {
          : shader == TEX_SHADOW ? 11 : 0;
 }
 
-// One triangle after setup (triangles.rs:55-171).  Lives in registers: the
-// raster pass recomputes it from the vertices' clip positions instead of
-// storing it.
+// One triangle after setup (triangles.rs:55-171).
 struct TriRec {
     float o[3], dx[3], dy[3];   // w_hom_origin, w_hom_dx, w_hom_dy
     float hz[3];                // verts_hom[i][2] after flip/reversal
     float vby[6];               // verts_by_y (screen x, y) x 3
     uint32_t xmin, xmax;        // bounds_clamped x, clamped to [0, W]
     uint32_t ymin, ymax;        // bounds_clamped y, clamped to [0, H]
-    uint32_t flags;             // bit 0: all three vertices pass the z clip
-    uint32_t sid[3];            // shaded-vertex id in clip / vary (request base + instance * n_vertices + vertex), post-reversal
+    uint32_t flags;             // TRI_ALL_IN | TRI_SWAPPED
+    uint32_t sid[3];            // shaded-vertex id in vary (request base + instance * n_vertices + vertex), post-reversal
 };
+
+enum TriFlags : uint32_t {
+    TRI_ALL_IN = 1u,   // all three vertices pass the z clip
+    TRI_SWAPPED = 2u,  // vertices 0 and 2 were exchanged (winding >= 0)
+};
+
+// Setup records written by k_setup for every kept triangle, read by k_raster.
+// rec0 (one 64-byte line): o[3], dx[3], dy[3], hz[3], xmin, xmax, ymin,
+//                          ymax | flags << 30
+// rec1 (32 bytes):         verts_by_y x, y x 3 (read only for the spans of
+//                          large triangles), 2 words unused
+constexpr uint32_t kRec0Quads = 4, kRec1Quads = 2;
 
 enum FrameFlags : uint32_t { FLAG_BIN_OVERFLOW = 1u };
 
@@ -58,7 +68,8 @@ struct View {
     uint32_t G;            // render_par band height (rows), H for render_seq
     uint32_t ntx, nty;     // tiles of the target
     uint32_t tile_base;    // first tile of this request in the launch's tile space
-    uint32_t sv_base;      // first shaded vertex of this request in clip / vary
+    uint32_t sv_base;      // first shaded vertex of this request in vary
+    uint32_t tri_base;     // first triangle of this request in rec0 / rec1
     uint32_t tile_cap;     // bin slots per tile of this request
     uint64_t bin_base;     // first bin slot of this request
     uint32_t *pixel;
@@ -89,7 +100,8 @@ struct FrameParams {
     float z_min, z_max;
     uint32_t cull, msaa_level, band_par;
     // per-launch scratch (device)
-    float4 *clip;          // clip position per shaded vertex
+    uint4 *rec0;           // kRec0Quads per triangle of the launch
+    uint4 *rec1;           // kRec1Quads per triangle of the launch
     float *vary;           // VertexData per shaded vertex
     Counters *ctr;
     uint32_t *tile_count;  // per tile of the launch, zero between frames

@@ -82,11 +82,9 @@ EUCHIP_DEV void for_each_tile(bool act, uint32_t tx0, uint32_t ty0, uint32_t nx,
 
 // ---------------------------------------------------------------------------
 // triangle setup (triangles.rs:55-171) of primitive t of request v, whole-target
-// band clamp.  CLIPBUF: read the vertices' clip positions from p.clip (written
-// by k_setup) instead of running the vertex shader.  Returns false when the
+// band clamp, from the vertex shader's clip positions.  Returns false when the
 // triangle is culled or its bounding box is empty.
 // ---------------------------------------------------------------------------
-template <bool CLIPBUF>
 EUCHIP_DEV bool tri_setup(const FrameParams &p, const View &v, uint32_t t, TriRec &R) {
     float hom[3][4], euc[3][3];
     uint32_t sid[3];
@@ -99,12 +97,7 @@ EUCHIP_DEV bool tri_setup(const FrameParams &p, const View &v, uint32_t t, TriRe
         const uint32_t vi = v.idx ? v.idx[k] : k;
         sid[i] = v.sv_base + inst * v.n_vertices + vi;
         float c[4];
-        if (CLIPBUF) {
-            const float4 q = p.clip[sid[i]];
-            c[0] = q.x, c[1] = q.y, c[2] = q.z, c[3] = q.w;
-        } else {
-            vertex_clip(p, v, inst, vi, c);
-        }
+        vertex_clip(p, v, inst, vi, c);
         hom[i][0] = c[0] * 1.0f;
         hom[i][1] = c[1] * flip1;
         hom[i][2] = c[2];
@@ -122,7 +115,8 @@ EUCHIP_DEV bool tri_setup(const FrameParams &p, const View &v, uint32_t t, TriRe
     const float winding = a0 * b1 - a1 * b0;
     if (p.cull == 1u && winding * 1.0f < 0.0f) return false;
     if (p.cull == 2u && winding * -1.0f < 0.0f) return false;
-    if (winding >= 0.0f) {
+    const bool swapped = winding >= 0.0f;
+    if (swapped) {
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const float tmp = hom[0][c];
@@ -204,7 +198,7 @@ EUCHIP_DEV bool tri_setup(const FrameParams &p, const View &v, uint32_t t, TriRe
     }
     R.vby[0] = ax, R.vby[1] = ay, R.vby[2] = bx, R.vby[3] = by, R.vby[4] = cx, R.vby[5] = cy;
     const bool allin = passes_z_clip(p, euc[0][2]) && passes_z_clip(p, euc[1][2]) && passes_z_clip(p, euc[2][2]);
-    R.flags = allin ? 1u : 0u;
+    R.flags = (allin ? TRI_ALL_IN : 0u) | (swapped ? TRI_SWAPPED : 0u);
     return R.xmin < R.xmax && R.ymin < R.ymax;
 }
 
@@ -233,6 +227,59 @@ EUCHIP_DEV const View &view_of(const FrameParams &p, uint32_t r) {
     else return p.v0;
 }
 
+// Setup records (layout in internal.hpp).
+EUCHIP_DEV void pack_rec(const TriRec &R, uint4 q0[kRec0Quads], uint4 q1[kRec1Quads]) {
+    q0[0] = make_uint4(__float_as_uint(R.o[0]), __float_as_uint(R.o[1]), __float_as_uint(R.o[2]), __float_as_uint(R.dx[0]));
+    q0[1] = make_uint4(__float_as_uint(R.dx[1]), __float_as_uint(R.dx[2]), __float_as_uint(R.dy[0]), __float_as_uint(R.dy[1]));
+    q0[2] = make_uint4(__float_as_uint(R.dy[2]), __float_as_uint(R.hz[0]), __float_as_uint(R.hz[1]), __float_as_uint(R.hz[2]));
+    q0[3] = make_uint4(R.xmin, R.xmax, R.ymin, R.ymax | (R.flags << 30));
+    q1[0] = make_uint4(__float_as_uint(R.vby[0]), __float_as_uint(R.vby[1]), __float_as_uint(R.vby[2]), __float_as_uint(R.vby[3]));
+    q1[1] = make_uint4(__float_as_uint(R.vby[4]), __float_as_uint(R.vby[5]), 0u, 0u);
+}
+
+// o, dx, dy, hz, bounds and flags of record `rid`.
+EUCHIP_DEV void load_rec0(const FrameParams &p, uint32_t rid, TriRec &R) {
+    const uint4 *q = p.rec0 + (size_t)rid * kRec0Quads;
+    const uint4 a = q[0], b = q[1], c = q[2], d = q[3];
+    R.o[0] = __uint_as_float(a.x), R.o[1] = __uint_as_float(a.y), R.o[2] = __uint_as_float(a.z);
+    R.dx[0] = __uint_as_float(a.w), R.dx[1] = __uint_as_float(b.x), R.dx[2] = __uint_as_float(b.y);
+    R.dy[0] = __uint_as_float(b.z), R.dy[1] = __uint_as_float(b.w), R.dy[2] = __uint_as_float(c.x);
+    R.hz[0] = __uint_as_float(c.y), R.hz[1] = __uint_as_float(c.z), R.hz[2] = __uint_as_float(c.w);
+    R.xmin = d.x, R.xmax = d.y, R.ymin = d.z, R.ymax = d.w & 0x3fffffffu, R.flags = d.w >> 30;
+}
+
+// verts_by_y of record `rid`.
+EUCHIP_DEV void load_rec1(const FrameParams &p, uint32_t rid, TriRec &R) {
+    const uint4 *q = p.rec1 + (size_t)rid * kRec1Quads;
+    const uint4 a = q[0], b = q[1];
+    R.vby[0] = __uint_as_float(a.x), R.vby[1] = __uint_as_float(a.y), R.vby[2] = __uint_as_float(a.z);
+    R.vby[3] = __uint_as_float(a.w), R.vby[4] = __uint_as_float(b.x), R.vby[5] = __uint_as_float(b.y);
+}
+
+// Shaded-vertex ids of primitive t's corners after the winding swap.
+EUCHIP_DEV void tri_sids(const View &v, uint32_t t, uint32_t flags, uint32_t sid[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const uint32_t e = 3u * t + (uint32_t)i;
+        const uint32_t inst = e / v.count;
+        const uint32_t k = e - inst * v.count;
+        sid[i] = v.sv_base + inst * v.n_vertices + (v.idx ? v.idx[k] : k);
+    }
+    if (flags & TRI_SWAPPED) {
+        const uint32_t s0 = sid[0];
+        sid[0] = sid[2];
+        sid[2] = s0;
+    }
+}
+
+// A triangle's rows all take the bounding-box span ("stupid version",
+// triangles.rs:229-233) when its whole-target box is small; band clamping can
+// only shrink the box, so then no row needs verts_by_y.
+EUCHIP_DEV bool all_rows_boxed(const TriRec &R) {
+    const uint32_t bw = R.xmax - R.xmin, bh = R.ymax - R.ymin;
+    return bw < 128u && bh < 128u && bw * bh < 128u;
+}
+
 constexpr uint32_t kHash = 1024;   // LDS hash slots (distinct tiles) per setup workgroup
 constexpr uint32_t kPairs = 2048;  // (tile, primitive) pairs grouped in LDS per workgroup
 
@@ -243,10 +290,22 @@ constexpr uint32_t kPairs = 2048;  // (tile, primitive) pairs grouped in LDS per
 // distinct tile costs one global atomic per workgroup; ranks inside the
 // group come from LDS.
 // ---------------------------------------------------------------------------
+// LDS of k_setup: the workgroup's records on their way out, then the binning tables.
+union SetupShared {
+    struct {
+        uint4 q0[256 * kRec0Quads];
+        uint4 q1[256 * kRec1Quads];
+        uint32_t kept[256];
+    } rec;
+    struct {
+        uint32_t hkey[kHash], hcnt[kHash], hbase[kHash];
+        uint32_t pl_prim[kPairs], pl_slot[kPairs], pl_local[kPairs];
+    } bin;
+};
+
 template <uint32_t SH, bool BATCH>
 __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
-    __shared__ uint32_t hkey[kHash], hcnt[kHash], hbase[kHash];
-    __shared__ uint32_t pl_prim[kPairs], pl_slot[kPairs], pl_local[kPairs];
+    __shared__ SetupShared sm;
     __shared__ uint32_t npl;
     const uint32_t tid = threadIdx.x;
     uint32_t r = 0, j = blockIdx.x;
@@ -256,16 +315,11 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
     }
     const View &v = view_of<BATCH>(p, r);
     const uint32_t t = j * 256u + tid;
-    for (uint32_t i = tid; i < kHash; i += 256u) hkey[i] = 0u, hcnt[i] = 0u;
-    if (tid == 0u) npl = 0u;
-    if (t < v.n_instances * v.n_vertices) {
-        const uint32_t inst = t / v.n_vertices;
-        const uint32_t vi = t - inst * v.n_vertices;
-        const size_t sv = (size_t)v.sv_base + t;
-        float c[4];
-        vertex_clip(p, v, inst, vi, c);
-        p.clip[sv] = make_float4(c[0], c[1], c[2], c[3]);
-        if constexpr (vd_count(SH) > 0) {
+    if constexpr (vd_count(SH) > 0) {
+        if (t < v.n_instances * v.n_vertices) {
+            const uint32_t inst = t / v.n_vertices;
+            const uint32_t vi = t - inst * v.n_vertices;
+            const size_t sv = (size_t)v.sv_base + t;
             float vd[VDN<SH>::N];
             vertex_data<SH>(p, v, inst, vi, vd);
 #pragma unroll
@@ -277,27 +331,53 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
     uint32_t tx0 = 0, ty0 = 0, nx = 0, ny = 0;
     if (t < v.n_tris) {
         TriRec R;
-        keep = tri_setup<false>(p, v, t, R);
+        keep = tri_setup(p, v, t, R);
         if (keep) {
             tx0 = R.xmin >> sx;
             ty0 = R.ymin >> sy;
             nx = ((R.xmax - 1u) >> sx) - tx0 + 1u;
             ny = ((R.ymax - 1u) >> sy) - ty0 + 1u;
+            uint4 q0[kRec0Quads], q1[kRec1Quads];
+            pack_rec(R, q0, q1);
+#pragma unroll
+            for (uint32_t k = 0; k < kRec0Quads; k++) sm.rec.q0[tid * kRec0Quads + k] = q0[k];
+#pragma unroll
+            for (uint32_t k = 0; k < kRec1Quads; k++) sm.rec.q1[tid * kRec1Quads + k] = q1[k];
         }
     }
+    sm.rec.kept[tid] = keep ? 1u : 0u;
+    __syncthreads();
+    // the workgroup's records are contiguous: write them out 16 bytes per lane,
+    // skipping the culled triangles' slots
+    {
+        const size_t base = (size_t)v.tri_base + (size_t)j * 256u;
+#pragma unroll
+        for (uint32_t k = 0; k < kRec0Quads; k++) {
+            const uint32_t q = k * 256u + tid;
+            if (sm.rec.kept[q / kRec0Quads]) p.rec0[base * kRec0Quads + q] = sm.rec.q0[q];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kRec1Quads; k++) {
+            const uint32_t q = k * 256u + tid;
+            if (sm.rec.kept[q / kRec1Quads]) p.rec1[base * kRec1Quads + q] = sm.rec.q1[q];
+        }
+    }
+    __syncthreads();  // the binning tables reuse the record staging
+    for (uint32_t i = tid; i < kHash; i += 256u) sm.bin.hkey[i] = 0u, sm.bin.hcnt[i] = 0u;
+    if (tid == 0u) npl = 0u;
     __syncthreads();
     const uint32_t tbase = v.tile_base;
     for_each_tile(keep, tx0, ty0, nx, ny, v.ntx, t, 0u, [&](uint32_t lt, uint32_t, uint32_t prim, uint32_t) {
         const uint32_t tile = tbase + lt;
         uint32_t h = (tile * 2654435761u) >> 22;  // 10-bit multiplicative hash
         for (uint32_t probe = 0; probe < 16u; probe++) {
-            const uint32_t old = atomicCAS(&hkey[h], 0u, tile + 1u);
+            const uint32_t old = atomicCAS(&sm.bin.hkey[h], 0u, tile + 1u);
             if (old == 0u || old == tile + 1u) {
                 const uint32_t k = atomicAdd(&npl, 1u);
                 if (k < kPairs) {
-                    pl_prim[k] = prim;
-                    pl_slot[k] = h;
-                    pl_local[k] = atomicAdd(&hcnt[h], 1u);
+                    sm.bin.pl_prim[k] = prim;
+                    sm.bin.pl_slot[k] = h;
+                    sm.bin.pl_local[k] = atomicAdd(&sm.bin.hcnt[h], 1u);
                     return;
                 }
                 break;
@@ -308,12 +388,12 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
     });
     __syncthreads();
     for (uint32_t i = tid; i < kHash; i += 256u)
-        if (hcnt[i]) hbase[i] = atomicAdd(p.tile_count + (hkey[i] - 1u), hcnt[i]);
+        if (sm.bin.hcnt[i]) sm.bin.hbase[i] = atomicAdd(p.tile_count + (sm.bin.hkey[i] - 1u), sm.bin.hcnt[i]);
     __syncthreads();
     const uint32_t np = npl < kPairs ? npl : kPairs;
     for (uint32_t k = tid; k < np; k += 256u) {
-        const uint32_t h = pl_slot[k];
-        bin_store(p, v, hkey[h] - 1u, hbase[h] + pl_local[k], pl_prim[k]);
+        const uint32_t h = sm.bin.pl_slot[k];
+        bin_store(p, v, sm.bin.hkey[h] - 1u, sm.bin.hbase[h] + sm.bin.pl_local[k], sm.bin.pl_prim[k]);
     }
 }
 
@@ -519,8 +599,9 @@ EUCHIP_DEV void raster_lane(const FrameParams &p, const TriRec &R, uint32_t prim
 // hand it to the pixel-parallel pass.
 constexpr uint32_t kLaneWork = 48;
 constexpr uint32_t kMaxTileH = 32;
+constexpr uint32_t kBigChunk = 64;  // large triangles staged in LDS at a time
 
-// Setup fields the pixel-parallel pass needs, kept in LDS.
+// Setup fields the pixel-parallel pass needs, staged in LDS.
 struct BigTri {
     float o[3], dx[3], dy[3], hz[3], vby[6];
     uint32_t xmin, xmax, ymin, ymax, flags, prim;
@@ -529,62 +610,77 @@ struct BigTri {
 struct TileShared {
     uint64_t key[256];              // winner key per pixel
     float d0s[256];                 // depth before the pass (the test's reference)
-    BigTri big[256];                // triangles handed to the pixel-parallel pass
+    uint32_t bigp[256];             // primitives handed to the pixel-parallel pass
+    BigTri big[kBigChunk];          // their setup, a chunk at a time
     uint32_t band[2 * kMaxTileH];   // [by0, by1) of the render_par band of each row
-    uint32_t nbig;
+    uint32_t nbig[2];               // entries of bigp, per batch parity
     uint32_t next;                  // tile taken from the dynamic queue
 };
 
-// One bin entry: rasterise it now (lane-parallel) or queue its setup for the
+// One bin entry: rasterise it now (lane-parallel) or queue it for the
 // pixel-parallel pass.
 EUCHIP_DEV void bin_entry(const FrameParams &p, const View &v, TileShared &sh, uint32_t prim, const TileGeom &g,
-                          uint32_t mode) {
+                          uint32_t mode, uint32_t par) {
+    const uint32_t rid = v.tri_base + prim;
     TriRec R;
-    tri_setup<true>(p, v, prim, R);
+    load_rec0(p, rid, R);
     const uint32_t ry0 = R.ymin > g.Y0 ? R.ymin : g.Y0;
     const uint32_t ry1 = R.ymax < g.Y0 + g.TH ? R.ymax : g.Y0 + g.TH;
     const uint32_t xe = R.xmax < g.X0 + g.TW ? R.xmax : g.X0 + g.TW;
     const uint32_t cost = (ry1 > ry0 ? ry1 - ry0 : 0u) * (xe > R.xmin ? xe - R.xmin : 0u);
     if (cost <= kLaneWork) {
+        if (!all_rows_boxed(R)) load_rec1(p, rid, R);
         raster_lane(p, R, prim, g, mode, sh.key, sh.d0s, sh.band);
     } else {
-        BigTri &B = sh.big[atomicAdd(&sh.nbig, 1u)];
-#pragma unroll
-        for (int i = 0; i < 3; i++) B.o[i] = R.o[i], B.dx[i] = R.dx[i], B.dy[i] = R.dy[i], B.hz[i] = R.hz[i];
-#pragma unroll
-        for (int i = 0; i < 6; i++) B.vby[i] = R.vby[i];
-        B.xmin = R.xmin, B.xmax = R.xmax, B.ymin = R.ymin, B.ymax = R.ymax, B.flags = R.flags, B.prim = prim;
+        sh.bigp[atomicAdd(&sh.nbig[par], 1u)] = prim;
     }
 }
 
-// The queued large triangles, one pixel per thread.
-EUCHIP_DEV void big_pass(const FrameParams &p, TileShared &sh, uint32_t tid, uint32_t px, uint32_t py, bool inside,
-                         uint32_t sx, uint32_t mode, float D0) {
-    const uint32_t nb = sh.nbig;
+// The nb queued large triangles, one pixel per thread; their setup is staged in
+// LDS kBigChunk at a time (every thread calls this with the same nb).
+EUCHIP_DEV void big_pass(const FrameParams &p, const View &v, TileShared &sh, uint32_t nb, uint32_t tid, uint32_t px,
+                         uint32_t py, bool inside, uint32_t sx, uint32_t mode, float D0) {
     const uint32_t r = tid >> sx;
-    for (uint32_t k = 0; k < nb; k++) {
-        const BigTri &B = sh.big[k];
-        if (!inside || py < B.ymin || py >= B.ymax) continue;
-        TriRec R;
+    for (uint32_t c0 = 0; c0 < nb; c0 += kBigChunk) {
+        const uint32_t n = nb - c0 < kBigChunk ? nb - c0 : kBigChunk;
+        if (tid < n) {
+            const uint32_t prim = sh.bigp[c0 + tid];
+            TriRec R;
+            load_rec0(p, v.tri_base + prim, R);
+            load_rec1(p, v.tri_base + prim, R);
+            BigTri &B = sh.big[tid];
 #pragma unroll
-        for (int i = 0; i < 3; i++) R.o[i] = B.o[i], R.dx[i] = B.dx[i], R.dy[i] = B.dy[i], R.hz[i] = B.hz[i];
+            for (int i = 0; i < 3; i++) B.o[i] = R.o[i], B.dx[i] = R.dx[i], B.dy[i] = R.dy[i], B.hz[i] = R.hz[i];
 #pragma unroll
-        for (int i = 0; i < 6; i++) R.vby[i] = B.vby[i];
-        R.xmin = B.xmin, R.xmax = B.xmax, R.ymin = B.ymin, R.ymax = B.ymax, R.flags = B.flags;
-        uint32_t x0, x1;
-        row_span(R, py, sh.band[2 * r], sh.band[2 * r + 1], x0, x1);
-        if (px < x0 || px >= x1) continue;
-        const float Y = (float)py, X0 = (float)x0;
-        float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
-        float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
-        float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
-        for (uint32_t x = x0; x < px; x++) {
-            w0 = w0 + R.dx[0];
-            w1 = w1 + R.dx[1];
-            w2 = w2 + R.dx[2];
+            for (int i = 0; i < 6; i++) B.vby[i] = R.vby[i];
+            B.xmin = R.xmin, B.xmax = R.xmax, B.ymin = R.ymin, B.ymax = R.ymax, B.flags = R.flags, B.prim = prim;
         }
-        float z;
-        if (frag_z(p, R, w0, w1, w2, z)) update_key(sh.key + tid, mode, p.depth_test, z, D0, B.prim);
+        __syncthreads();
+        for (uint32_t k = 0; k < n; k++) {
+            const BigTri &B = sh.big[k];
+            if (!inside || py < B.ymin || py >= B.ymax) continue;
+            TriRec R;
+#pragma unroll
+            for (int i = 0; i < 3; i++) R.o[i] = B.o[i], R.dx[i] = B.dx[i], R.dy[i] = B.dy[i], R.hz[i] = B.hz[i];
+#pragma unroll
+            for (int i = 0; i < 6; i++) R.vby[i] = B.vby[i];
+            R.xmin = B.xmin, R.xmax = B.xmax, R.ymin = B.ymin, R.ymax = B.ymax, R.flags = B.flags;
+            uint32_t x0, x1;
+            row_span(R, py, sh.band[2 * r], sh.band[2 * r + 1], x0, x1);
+            if (px < x0 || px >= x1) continue;
+            const float Y = (float)py, X0 = (float)x0;
+            float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
+            float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
+            float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
+            for (uint32_t x = x0; x < px; x++) {
+                w0 = w0 + R.dx[0];
+                w1 = w1 + R.dx[1];
+                w2 = w2 + R.dx[2];
+            }
+            float z;
+            if (frag_z(p, R, w0, w1, w2, z)) update_key(sh.key + tid, mode, p.depth_test, z, D0, B.prim);
+        }
+        __syncthreads();
     }
 }
 
@@ -634,7 +730,7 @@ __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
         if (inside && uses_depth) D0 = v.clear_depth_en ? v.clear_depth : v.depth[pidx];
         sh.key[tid] = init_key(mode, D0);
         sh.d0s[tid] = D0;
-        if (tid == 0u) sh.nbig = 0u;
+        if (tid == 0u) sh.nbig[0] = sh.nbig[1] = 0u;
         if (tid < g.TH) {
             uint32_t by0, by1;
             band_of(p, v, g.Y0 + tid, by0, by1);
@@ -652,19 +748,22 @@ __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
         const uint32_t nb = cnt < cap ? cnt : cap;
         const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
         const uint32_t nscan = nb + (cnt > cap ? novf : 0u);
+        // nbig[par] counts this batch's large triangles; the other counter is
+        // cleared during the batch (nobody reads it then) for the next one
+        uint32_t par = 0;
         for (uint32_t b = 0; b < nscan; b += 256u) {
             const uint32_t i = b + tid;
+            if (tid == 0u) sh.nbig[par ^ 1u] = 0u;
             if (i < nb) {
-                bin_entry(p, v, sh, bin[i], g, mode);
+                bin_entry(p, v, sh, bin[i], g, mode, par);
             } else if (i < nscan) {
                 const uint2 e = p.ovf[i - nb];
-                if (e.x == tile) bin_entry(p, v, sh, e.y, g, mode);
+                if (e.x == tile) bin_entry(p, v, sh, e.y, g, mode, par);
             }
             __syncthreads();
-            big_pass(p, sh, tid, px, py, inside, sx, mode, D0);
-            __syncthreads();
-            if (tid == 0u) sh.nbig = 0u;
-            __syncthreads();
+            const uint32_t nbig = sh.nbig[par];
+            if (nbig) big_pass(p, v, sh, nbig, tid, px, py, inside, sx, mode, D0);
+            par ^= 1u;
         }
         if (inside) {
             const uint64_t kk = sh.key[tid];
@@ -684,7 +783,8 @@ __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
             if (v.pixel) {
                 if (p.pixel_write && has && SH != DEPTH_ONLY) {
                     TriRec R;
-                    tri_setup<true>(p, v, prim, R);
+                    load_rec0(p, v.tri_base + prim, R);
+                    tri_sids(v, prim, R.flags, R.sid);
                     v.pixel[pidx] = shade_pixel<SH>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
                                                     v.clear_color_en ? v.clear_color : 0u);
                 } else if (v.clear_color_en) {

@@ -50,7 +50,7 @@ struct euchip_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, Stat> stats;
     // per-frame scratch, grown on demand
-    DevArray<float4> clip;
+    DevArray<uint4> rec0, rec1;
     DevArray<float> vary;
     DevArray<uint32_t> tile_count, bins;
     DevArray<uint2> ovf;
@@ -107,7 +107,7 @@ struct euchip_batch {
     DevArray<View> views;
     DevArray<uint2> blk;
     DevArray<uint32_t> tile_req;
-    DevArray<float4> clip;
+    DevArray<uint4> rec0, rec1;
     DevArray<float> vary;
     DevArray<uint32_t> tile_count, bins, ctrbuf;  // ctrbuf: Counters, then req_max per active request
     DevArray<uint2> ovf;
@@ -421,7 +421,8 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     p.n_views = 1;
 
     const size_t nsv = P.n_sv;
-    if ((s = ensure(ctx, ctx->clip, std::max<size_t>(nsv, 1), "clip positions", ctx->stream))) return s;
+    if ((s = ensure(ctx, ctx->rec0, std::max<size_t>(P.n_tris * kRec0Quads, 1), "setup records", ctx->stream))) return s;
+    if ((s = ensure(ctx, ctx->rec1, std::max<size_t>(P.n_tris * kRec1Quads, 1), "setup records", ctx->stream))) return s;
     if ((s = ensure(ctx, ctx->vary, std::max<size_t>(nsv * vd_count(d->shader), 1), "vertex data", ctx->stream)))
         return s;
     const bool fresh_tiles = ctx->tile_count.cap < ntiles;
@@ -432,7 +433,8 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
 
     for (int attempt = 0;; attempt++) {
         if ((s = ensure(ctx, ctx->bins, ntiles * ctx->tile_cap, "bins", ctx->stream))) return s;
-        p.clip = ctx->clip.ptr;
+        p.rec0 = ctx->rec0.ptr;
+        p.rec1 = ctx->rec1.ptr;
         p.vary = ctx->vary.ptr;
         p.ctr = ctx->counters;
         p.req_max = &ctx->counters->max_count;
@@ -503,6 +505,7 @@ euchip_status build_tables(euchip_batch *b) {
         if (caps[k] == 0) caps[k] = initial_cap(P.n_tris, nt);
         v.tile_base = (uint32_t)tiles;
         v.sv_base = (uint32_t)sv;
+        v.tri_base = (uint32_t)tris;
         v.tile_cap = caps[k];
         v.bin_base = bins;
         hv[k] = v;
@@ -513,7 +516,7 @@ euchip_status build_tables(euchip_batch *b) {
         bins += nt * caps[k];
         sv += P.n_sv;
         tris += P.n_tris;
-        if (tiles >= (1ull << 32) || sv >= (1ull << 32) || hblk.size() >= (1ull << 31))
+        if (tiles >= (1ull << 32) || sv >= (1ull << 32) || tris >= (1ull << 32) || hblk.size() >= (1ull << 31))
             return fail(ctx, EUCHIP_INVALID_ARG, "batch too large for 32-bit tile / vertex ids");
     }
     euchip_status s;
@@ -521,7 +524,8 @@ euchip_status build_tables(euchip_batch *b) {
     if ((s = ensure(ctx, b->views, std::max<size_t>(na, 1), "batch views", b->stream))) return s;
     if ((s = ensure(ctx, b->blk, std::max<size_t>(hblk.size(), 1), "batch blocks", b->stream))) return s;
     if ((s = ensure(ctx, b->tile_req, std::max<size_t>(tiles, 1), "batch tiles", b->stream))) return s;
-    if ((s = ensure(ctx, b->clip, std::max<size_t>(sv, 1), "batch clip positions", b->stream))) return s;
+    if ((s = ensure(ctx, b->rec0, std::max<size_t>(tris * kRec0Quads, 1), "batch setup records", b->stream))) return s;
+    if ((s = ensure(ctx, b->rec1, std::max<size_t>(tris * kRec1Quads, 1), "batch setup records", b->stream))) return s;
     if ((s = ensure(ctx, b->vary, std::max<size_t>(sv * vdc, 1), "batch vertex data", b->stream))) return s;
     if ((s = ensure(ctx, b->tile_count, std::max<size_t>(tiles, 1), "batch tile counts", b->stream))) return s;
     if ((s = ensure(ctx, b->bins, std::max<size_t>(bins, 1), "batch bins", b->stream))) return s;
@@ -545,7 +549,8 @@ euchip_status build_tables(euchip_batch *b) {
     FrameParams &p = b->p;
     memset(&p, 0, sizeof(p));
     fill_pipe(p, &b->desc);
-    p.clip = b->clip.ptr;
+    p.rec0 = b->rec0.ptr;
+    p.rec1 = b->rec1.ptr;
     p.vary = b->vary.ptr;
     p.ctr = (Counters *)b->ctrbuf.ptr;
     p.req_max = b->ctrbuf.ptr + sizeof(Counters) / 4;
@@ -660,7 +665,8 @@ void free_batch_memory(euchip_batch *b) {
     release(ctx, b->views);
     release(ctx, b->blk);
     release(ctx, b->tile_req);
-    release(ctx, b->clip);
+    release(ctx, b->rec0);
+    release(ctx, b->rec1);
     release(ctx, b->vary);
     release(ctx, b->tile_count);
     release(ctx, b->bins);
@@ -680,8 +686,8 @@ size_t request_scratch_estimate(const euchip_pipeline_desc *d, const euchip_mesh
     const uint64_t n_sv = (uint64_t)m->n_instances * m->n_vertices;
     const uint64_t nt = (uint64_t)((W + 31) / 32) * ((H + 7) / 8);
     const uint64_t nb = (std::max<uint64_t>(n_sv, n_tris) + 255) / 256;
-    return sizeof(View) + nb * sizeof(uint2) + nt * 8 + n_sv * (16 + 4 * vd_count(d->shader)) +
-           nt * initial_cap(n_tris, nt) * 4 + 4 + 8 * n_tris;
+    return sizeof(View) + nb * sizeof(uint2) + nt * 8 + n_sv * 4 * vd_count(d->shader) +
+           n_tris * 16 * (kRec0Quads + kRec1Quads) + nt * initial_cap(n_tris, nt) * 4 + 4 + 8 * n_tris;
 }
 
 size_t batch_fixed_bytes() { return sizeof(Counters) + 8 * ((size_t)1 << 20); }
@@ -737,7 +743,8 @@ euchip_status euchip_ctx_destroy(euchip_ctx *ctx) {
         (void)hipEventDestroy(p.b);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-    release(ctx, ctx->clip);
+    release(ctx, ctx->rec0);
+    release(ctx, ctx->rec1);
     release(ctx, ctx->vary);
     release(ctx, ctx->tile_count);
     release(ctx, ctx->bins);
@@ -1080,7 +1087,8 @@ void *euchip_batch_stream(euchip_batch *b) { return b ? (void *)b->stream : null
 
 size_t euchip_batch_scratch_bytes(const euchip_batch *b) {
     if (!b) return 0;
-    return b->views.cap * sizeof(View) + b->blk.cap * sizeof(uint2) + b->tile_req.cap * 4 + b->clip.cap * 16 +
+    return b->views.cap * sizeof(View) + b->blk.cap * sizeof(uint2) + b->tile_req.cap * 4 + b->rec0.cap * 16 +
+           b->rec1.cap * 16 +
            b->vary.cap * 4 + b->tile_count.cap * 4 + b->bins.cap * 4 + b->ctrbuf.cap * 4 + b->ovf.cap * 8;
 }
 

@@ -58,3 +58,14 @@ def test_null_arguments_are_rejected():
     assert L.euchip_render(None, None, None, None, None, None) == 3
     assert L.euchip_buffer_destroy(None) == 3
     assert L.euchip_mesh_destroy(None) == 3
+
+
+def test_cpp_mirror_header_compiles(tmp_path):
+    """include/euchip.hpp (the C++ mirror of euc's Pipeline surface) is self-contained C++17."""
+    import subprocess
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "euchip.hpp"\nint main() { return euc::DepthMode::less_write().uses_depth() ? 0 : 1; }\n')
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", inc, str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
