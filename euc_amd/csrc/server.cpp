@@ -42,7 +42,6 @@ struct Slot {
     bool busy = false;
     std::vector<Queued> reqs;
     std::vector<Target> targets;
-    size_t scratch_need = 0;  // estimate of the running batch's scratch
 };
 
 size_t target_bytes(uint32_t w, uint32_t h) { return (size_t)w * h * 8; }
@@ -135,8 +134,38 @@ bool any_busy(const euchip_server *s) {
     return false;
 }
 
+// Give back memory nothing is using: pooled targets, and the scratch of idle
+// slots other than `keep` (their batches are recreated empty).
+void free_transient(euchip_server *s, const Slot *keep) {
+    for (auto &kv : s->pool) destroy_target(kv.second);
+    s->pool.clear();
+    for (Slot &sl : s->slots) {
+        if (&sl == keep || sl.busy || euchip_batch_scratch_bytes(sl.batch) == 0) continue;
+        euchip_batch *fresh = nullptr;
+        if (euchip_batch_create(s->ctx, &fresh) != EUCHIP_OK) continue;
+        euchip_batch_destroy(sl.batch);
+        sl.batch = fresh;
+    }
+}
+
+// The head request cannot be admitted now.  It waits while memory will come
+// back (a batch running, targets the caller has not released); otherwise idle
+// memory is freed once and, if it still does not fit, it is rejected.
+bool head_blocked(euchip_server *s, Slot &sl, bool &freed, euchip_status why) {
+    if (any_busy(s) || !s->out.empty() || !s->done.empty()) return false;
+    if (!freed) {
+        free_transient(s, &sl);
+        freed = true;
+        return true;  // try again
+    }
+    complete(s, s->queue.front().ticket, why, Target());
+    s->queue.pop_front();
+    s->st.rejected++;
+    return true;
+}
+
 // Fill one free slot from the queue; false when nothing more can be admitted now.
-bool admit(euchip_server *s, Slot &sl) {
+bool admit(euchip_server *s, Slot &sl, bool &freed) {
     std::vector<Queued> take;
     std::vector<Target> tg;
     std::vector<bool> fresh;  // target must be allocated
@@ -156,22 +185,14 @@ bool admit(euchip_server *s, Slot &sl) {
             break;
         }
         take.push_back(q);
-        s->queue.pop_front();
-        t.w = q.w, t.h = q.h;
+        s->queue.pop_front();  // q is gone: use take.back()
+        t.w = take.back().w, t.h = take.back().h;
         tg.push_back(t);
         fresh.push_back(!pooled);
         scratch = sc;
         new_bytes += add;
     }
-    if (take.empty()) {
-        if (!s->queue.empty() && !any_busy(s)) {  // the head request alone can never fit
-            complete(s, s->queue.front().ticket, EUCHIP_OVER_BUDGET, Target());
-            s->queue.pop_front();
-            s->st.rejected++;
-            return true;
-        }
-        return false;
-    }
+    if (take.empty()) return !s->queue.empty() && head_blocked(s, sl, freed, EUCHIP_OVER_BUDGET);
     euchip_status st = EUCHIP_OK;
     for (size_t i = 0; i < tg.size() && !st; i++) {
         if (!fresh[i]) continue;
@@ -202,13 +223,7 @@ bool admit(euchip_server *s, Slot &sl) {
             s->batch_cap = std::max<uint32_t>(1, s->batch_cap / 2);
             return !any_busy(s);  // retry now only when nothing will free memory
         }
-        if (!any_busy(s)) {
-            complete(s, s->queue.front().ticket, st, Target());
-            s->queue.pop_front();
-            s->st.rejected++;
-            return true;
-        }
-        return false;
+        return head_blocked(s, sl, freed, st);
     }
     if (st) {  // a request the batch refuses (invalid arguments): fail the whole group
         for (size_t i = 0; i < take.size(); i++) {
@@ -288,9 +303,10 @@ euchip_status euchip_server_submit(euchip_server *s, const euchip_serve_request 
 euchip_status euchip_server_poll(euchip_server *s, euchip_completion *out, uint32_t max, uint32_t *n_out) {
     if (!s) return EUCHIP_INVALID_ARG;
     retire(s);
+    bool freed = false;
     for (Slot &sl : s->slots) {
         while (!sl.busy && !s->queue.empty())
-            if (!admit(s, sl)) break;
+            if (!admit(s, sl, freed)) break;
     }
     uint32_t n = 0;
     while (n < max && out && !s->done.empty()) {

@@ -55,7 +55,7 @@ struct CoordinateMode {
     std::optional<std::pair<float, float>> z_clip_range = std::make_pair(0.0f, 1.0f);
     static CoordinateMode opengl() { return {Handedness::Right, YAxisDirection::Up, std::make_pair(-1.0f, 1.0f)}; }
     static CoordinateMode vulkan() { return {Handedness::Left, YAxisDirection::Down, std::make_pair(0.0f, 1.0f)}; }
-    static CoordinateMode metal() { return {Handedness::Left, YAxisDirection::Up, std::make_pair(0.0f, 1.0f)}; }
+    static CoordinateMode metal() { return {Handedness::Right, YAxisDirection::Down, std::make_pair(0.0f, 1.0f)}; }
     static CoordinateMode directx() { return {Handedness::Left, YAxisDirection::Up, std::make_pair(0.0f, 1.0f)}; }
     CoordinateMode without_z_clip() const {
         CoordinateMode m = *this;

@@ -85,7 +85,7 @@ def test_serving_budget_admission():
         srv = Server(ctx, DESC, budget_bytes=budget, max_batch=16, slots=2)
         res = run_poisson(srv, items, rate_hz=50000.0, seed=5, keep=lambda i: True)
         st = res.stats
-        assert all(s == 0 for s in res.statuses), res.statuses
+        assert all(s == 0 for s in res.statuses), [(i, s) for i, s in enumerate(res.statuses) if s != 0]
         assert st["peak_bytes"] <= budget
         assert st["batches"] >= 3  # the budget split the burst
         for i, (rc, rd) in enumerate(ref):
