@@ -280,6 +280,37 @@ EUCHIP_DEV bool all_rows_boxed(const TriRec &R) {
     return bw < 128u && bh < 128u && bw * bh < 128u;
 }
 
+// Fragment test of one pixel (triangles.rs:262-301): weights, coverage, z, z-clip.
+EUCHIP_DEV bool frag_z(const FrameParams &p, const TriRec &R, float w0, float w1, float w2, float &z) {
+    const float u2 = (w2 - w0) - w1;
+    if (!(w0 >= 0.0f && w1 >= 0.0f && u2 >= 0.0f)) return false;
+    z = (R.hz[0] * w0 + R.hz[1] * w1) + R.hz[2] * u2;
+    return (R.flags & TRI_ALL_IN) || passes_z_clip(p, z);
+}
+
+// Triangles whose box has at most this many pixels are tested for coverage in
+// setup; one that covers no pixel corner emits no fragment and is not binned.
+constexpr uint32_t kCoverTestArea = 8;
+
+// Does a boxed triangle (all_rows_boxed) cover any pixel of its box, with the
+// raster pass's arithmetic (rows walked from xmin, triangles.rs:257-301)?
+EUCHIP_DEV bool covers_any(const FrameParams &p, const TriRec &R) {
+    for (uint32_t y = R.ymin; y < R.ymax; y++) {
+        const float Y = (float)y, X0 = (float)R.xmin;
+        float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
+        float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
+        float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
+        for (uint32_t x = R.xmin; x < R.xmax; x++) {
+            float z;
+            if (frag_z(p, R, w0, w1, w2, z)) return true;
+            w0 = w0 + R.dx[0];
+            w1 = w1 + R.dx[1];
+            w2 = w2 + R.dx[2];
+        }
+    }
+    return false;
+}
+
 constexpr uint32_t kHash = 1024;   // LDS hash slots (distinct tiles) per setup workgroup
 constexpr uint32_t kPairs = 2048;  // (tile, primitive) pairs grouped in LDS per workgroup
 
@@ -332,6 +363,8 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
     if (t < v.n_tris) {
         TriRec R;
         keep = tri_setup(p, v, t, R);
+        if (keep && all_rows_boxed(R) && (R.xmax - R.xmin) * (R.ymax - R.ymin) <= kCoverTestArea)
+            keep = covers_any(p, R);
         if (keep) {
             tx0 = R.xmin >> sx;
             ty0 = R.ymin >> sy;
@@ -546,14 +579,6 @@ EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const View &v, const TriRe
     return blend_shader<SH>(old, frag);
 }
 
-// Fragment test of one pixel (triangles.rs:262-301): weights, coverage, z, z-clip.
-EUCHIP_DEV bool frag_z(const FrameParams &p, const TriRec &R, float w0, float w1, float w2, float &z) {
-    const float u2 = (w2 - w0) - w1;
-    if (!(w0 >= 0.0f && w1 >= 0.0f && u2 >= 0.0f)) return false;
-    z = (R.hz[0] * w0 + R.hz[1] * w1) + R.hz[2] * u2;
-    return (R.flags & 1u) || passes_z_clip(p, z);
-}
-
 // Tile geometry: TW x TH pixels at (X0, Y0); TW = 1 << sx, TH = 256 >> sx.
 struct TileGeom {
     uint32_t X0, Y0, sx, TW, TH;
@@ -738,6 +763,7 @@ __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
             sh.band[2 * tid + 1] = by1;
         }
         const uint32_t cap = v.tile_cap;
+        const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
         const uint32_t cnt = p.tile_count[tile];
         __syncthreads();
         if (tid == 0u) {
@@ -746,7 +772,6 @@ __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
         }
         // this tile's bin, then (only when it overflowed) its entries of the overflow list
         const uint32_t nb = cnt < cap ? cnt : cap;
-        const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
         const uint32_t nscan = nb + (cnt > cap ? novf : 0u);
         // nbig[par] counts this batch's large triangles; the other counter is
         // cleared during the batch (nobody reads it then) for the next one
