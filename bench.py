@@ -28,6 +28,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-probe factor), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        k = t["kernels"][kernel]
+        return {"bytes": k["fetch_bytes"] + k["write_bytes"], "fetch_bytes": k["fetch_bytes"],
+                "write_bytes": k["write_bytes"], "source": "profiles/traffic.json"}
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def parse():
@@ -455,6 +469,7 @@ def main():
     frames_per_s = 1e3 / ms_per_step
     value = frags * frames_per_s if frags else None
     achieved = ab["raster"] / (raster_us * 1e-6) / 1e9 if raster_us == raster_us else None
+    traffic = measured_traffic("k_raster<PHONG>" if args.shader == "phong" else "k_raster<GOURAUD>")
     line = {
         "metric": "shaded frags/s (1080p Phong, 4x65536-tri torus)",
         "value": value,
@@ -472,7 +487,8 @@ def main():
         "kernels_us": kernels,
         "roofline": {"bound": "hbm", "kernel": "raster", "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBPS) if achieved else None,
-                     "traffic": None, "algorithmic_bytes_per_launch": ab["raster"]},
+                     "traffic": traffic["bytes"] if traffic else None, "traffic_detail": traffic,
+                     "algorithmic_bytes_per_launch": ab["raster"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -584,50 +584,43 @@ struct TileGeom {
     uint32_t X0, Y0, sx, TW, TH;
 };
 
-// A lane rasterises one triangle over the rows it has in this tile, walking
-// each row from the row start exactly as the reference does.
-EUCHIP_DEV void raster_lane(const FrameParams &p, const TriRec &R, uint32_t prim, const TileGeom &g, uint32_t mode,
-                            uint64_t *key, const float *d0s, const uint32_t *band) {
-    const uint32_t ry0 = R.ymin > g.Y0 ? R.ymin : g.Y0;
-    const uint32_t ry1 = R.ymax < g.Y0 + g.TH ? R.ymax : g.Y0 + g.TH;
-    for (uint32_t y = ry0; y < ry1; y++) {
-        uint32_t x0, x1;
-        const uint32_t r = y - g.Y0;
-        row_span(R, y, band[2 * r], band[2 * r + 1], x0, x1);
-        const uint32_t xs = x0 > g.X0 ? x0 : g.X0;
-        const uint32_t xe = x1 < g.X0 + g.TW ? x1 : g.X0 + g.TW;
-        if (xs >= xe) continue;
-        const float Y = (float)y, X0 = (float)x0;
-        float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
-        float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
-        float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
-        for (uint32_t x = x0; x < xs; x++) {
-            w0 = w0 + R.dx[0];
-            w1 = w1 + R.dx[1];
-            w2 = w2 + R.dx[2];
+// One row y of a triangle inside the tile: the row span (triangles.rs:219-254),
+// then the walk from the row start exactly as the reference does
+// (triangles.rs:257-301), testing the tile's pixels of the span.
+EUCHIP_DEV void raster_row(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t y, const TileGeom &g,
+                           uint32_t mode, uint64_t *key, const float *d0s, const uint32_t *band) {
+    uint32_t x0, x1;
+    const uint32_t r = y - g.Y0;
+    row_span(R, y, band[2 * r], band[2 * r + 1], x0, x1);
+    const uint32_t xs = x0 > g.X0 ? x0 : g.X0;
+    const uint32_t xe = x1 < g.X0 + g.TW ? x1 : g.X0 + g.TW;
+    if (xs >= xe) return;
+    const float Y = (float)y, X0 = (float)x0;
+    float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
+    float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
+    float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
+    for (uint32_t x = x0; x < xs; x++) {
+        w0 = w0 + R.dx[0];
+        w1 = w1 + R.dx[1];
+        w2 = w2 + R.dx[2];
+    }
+    const uint32_t row = r << g.sx;
+    for (uint32_t x = xs; x < xe; x++) {
+        float z;
+        if (frag_z(p, R, w0, w1, w2, z)) {
+            const uint32_t pi = row + (x - g.X0);
+            update_key(key + pi, mode, p.depth_test, z, d0s[pi], prim);
         }
-        const uint32_t row = r << g.sx;
-        for (uint32_t x = xs; x < xe; x++) {
-            float z;
-            if (frag_z(p, R, w0, w1, w2, z)) {
-                const uint32_t pi = row + (x - g.X0);
-                update_key(key + pi, mode, p.depth_test, z, d0s[pi], prim);
-            }
-            w0 = w0 + R.dx[0];
-            w1 = w1 + R.dx[1];
-            w2 = w2 + R.dx[2];
-        }
+        w0 = w0 + R.dx[0];
+        w1 = w1 + R.dx[1];
+        w2 = w2 + R.dx[2];
     }
 }
 
-// Lanes whose triangle would cost more than this many row steps in the tile
-// hand it to the pixel-parallel pass.
-constexpr uint32_t kLaneWork = 48;
 constexpr uint32_t kMaxTileH = 32;
-constexpr uint32_t kBigChunk = 64;  // large triangles staged in LDS at a time
 
-// Setup fields the pixel-parallel pass needs, staged in LDS.
-struct BigTri {
+// Setup of one bin entry of the current batch, kept in LDS (24 words).
+struct TriLds {
     float o[3], dx[3], dy[3], hz[3], vby[6];
     uint32_t xmin, xmax, ymin, ymax, flags, prim;
 };
@@ -635,78 +628,77 @@ struct BigTri {
 struct TileShared {
     uint64_t key[256];              // winner key per pixel
     float d0s[256];                 // depth before the pass (the test's reference)
-    uint32_t bigp[256];             // primitives handed to the pixel-parallel pass
-    BigTri big[kBigChunk];          // their setup, a chunk at a time
+    TriLds tri[256];                // the batch's triangles
+    uint32_t rows_incl[256];        // inclusive prefix sum of their row counts in the tile
+    uint32_t wave_rows[4];          // row counts per wave (prefix-sum carry)
     uint32_t band[2 * kMaxTileH];   // [by0, by1) of the render_par band of each row
-    uint32_t nbig[2];               // entries of bigp, per batch parity
     uint32_t next;                  // tile taken from the dynamic queue
 };
 
-// One bin entry: rasterise it now (lane-parallel) or queue it for the
-// pixel-parallel pass.
-EUCHIP_DEV void bin_entry(const FrameParams &p, const View &v, TileShared &sh, uint32_t prim, const TileGeom &g,
-                          uint32_t mode, uint32_t par) {
-    const uint32_t rid = v.tri_base + prim;
+// Rows of bin entry `prim` inside the tile, its setup kept in sh.tri[slot].
+EUCHIP_DEV uint32_t stage_entry(const FrameParams &p, const View &v, TileShared &sh, uint32_t slot, uint32_t prim,
+                                const TileGeom &g) {
     TriRec R;
-    load_rec0(p, rid, R);
+    load_rec0(p, v.tri_base + prim, R);
     const uint32_t ry0 = R.ymin > g.Y0 ? R.ymin : g.Y0;
     const uint32_t ry1 = R.ymax < g.Y0 + g.TH ? R.ymax : g.Y0 + g.TH;
+    const uint32_t xs = R.xmin > g.X0 ? R.xmin : g.X0;
     const uint32_t xe = R.xmax < g.X0 + g.TW ? R.xmax : g.X0 + g.TW;
-    const uint32_t cost = (ry1 > ry0 ? ry1 - ry0 : 0u) * (xe > R.xmin ? xe - R.xmin : 0u);
-    if (cost <= kLaneWork) {
-        if (!all_rows_boxed(R)) load_rec1(p, rid, R);
-        raster_lane(p, R, prim, g, mode, sh.key, sh.d0s, sh.band);
+    if (ry0 >= ry1 || xs >= xe) return 0u;
+    if (!all_rows_boxed(R)) {
+        load_rec1(p, v.tri_base + prim, R);
     } else {
-        sh.bigp[atomicAdd(&sh.nbig[par], 1u)] = prim;
+#pragma unroll
+        for (int i = 0; i < 6; i++) R.vby[i] = 0.0f;  // boxed rows never read verts_by_y
     }
+    TriLds &T = sh.tri[slot];
+#pragma unroll
+    for (int i = 0; i < 3; i++) T.o[i] = R.o[i], T.dx[i] = R.dx[i], T.dy[i] = R.dy[i], T.hz[i] = R.hz[i];
+#pragma unroll
+    for (int i = 0; i < 6; i++) T.vby[i] = R.vby[i];
+    T.xmin = R.xmin, T.xmax = R.xmax, T.ymin = R.ymin, T.ymax = R.ymax, T.flags = R.flags, T.prim = prim;
+    return ry1 - ry0;
 }
 
-// The nb queued large triangles, one pixel per thread; their setup is staged in
-// LDS kBigChunk at a time (every thread calls this with the same nb).
-EUCHIP_DEV void big_pass(const FrameParams &p, const View &v, TileShared &sh, uint32_t nb, uint32_t tid, uint32_t px,
-                         uint32_t py, bool inside, uint32_t sx, uint32_t mode, float D0) {
-    const uint32_t r = tid >> sx;
-    for (uint32_t c0 = 0; c0 < nb; c0 += kBigChunk) {
-        const uint32_t n = nb - c0 < kBigChunk ? nb - c0 : kBigChunk;
-        if (tid < n) {
-            const uint32_t prim = sh.bigp[c0 + tid];
-            TriRec R;
-            load_rec0(p, v.tri_base + prim, R);
-            load_rec1(p, v.tri_base + prim, R);
-            BigTri &B = sh.big[tid];
-#pragma unroll
-            for (int i = 0; i < 3; i++) B.o[i] = R.o[i], B.dx[i] = R.dx[i], B.dy[i] = R.dy[i], B.hz[i] = R.hz[i];
-#pragma unroll
-            for (int i = 0; i < 6; i++) B.vby[i] = R.vby[i];
-            B.xmin = R.xmin, B.xmax = R.xmax, B.ymin = R.ymin, B.ymax = R.ymax, B.flags = R.flags, B.prim = prim;
+// Inclusive prefix sum over the workgroup (4 waves); every thread must call it.
+EUCHIP_DEV uint32_t block_inclusive_sum(TileShared &sh, uint32_t tid, uint32_t v) {
+    const uint32_t incl = wave_inclusive_sum(v);
+    if ((tid & 63u) == 63u) sh.wave_rows[tid >> 6] = incl;
+    __syncthreads();
+    uint32_t carry = 0;
+    for (uint32_t w = 0; w < (tid >> 6); w++) carry += sh.wave_rows[w];
+    return incl + carry;
+}
+
+// One batch of up to 256 bin entries: each lane stages its entry's setup in LDS,
+// then the (entry, row) pairs of the batch are spread over all lanes so no
+// lane walks much more than one row.
+EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh, uint32_t tid, bool valid,
+                             uint32_t prim, const TileGeom &g, uint32_t mode) {
+    const uint32_t rows = valid ? stage_entry(p, v, sh, tid, prim, g) : 0u;
+    const uint32_t incl = block_inclusive_sum(sh, tid, rows);
+    sh.rows_incl[tid] = incl;
+    __syncthreads();
+    const uint32_t total = sh.rows_incl[255];
+    for (uint32_t k = tid; k < total; k += 256u) {
+        uint32_t lo = 0, hi = 255;  // first entry whose inclusive sum exceeds k
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sh.rows_incl[mid] > k) hi = mid;
+            else lo = mid + 1u;
         }
-        __syncthreads();
-        for (uint32_t k = 0; k < n; k++) {
-            const BigTri &B = sh.big[k];
-            if (!inside || py < B.ymin || py >= B.ymax) continue;
-            TriRec R;
+        const uint32_t base = lo ? sh.rows_incl[lo - 1u] : 0u;
+        const TriLds &T = sh.tri[lo];
+        TriRec R;
 #pragma unroll
-            for (int i = 0; i < 3; i++) R.o[i] = B.o[i], R.dx[i] = B.dx[i], R.dy[i] = B.dy[i], R.hz[i] = B.hz[i];
+        for (int i = 0; i < 3; i++) R.o[i] = T.o[i], R.dx[i] = T.dx[i], R.dy[i] = T.dy[i], R.hz[i] = T.hz[i];
 #pragma unroll
-            for (int i = 0; i < 6; i++) R.vby[i] = B.vby[i];
-            R.xmin = B.xmin, R.xmax = B.xmax, R.ymin = B.ymin, R.ymax = B.ymax, R.flags = B.flags;
-            uint32_t x0, x1;
-            row_span(R, py, sh.band[2 * r], sh.band[2 * r + 1], x0, x1);
-            if (px < x0 || px >= x1) continue;
-            const float Y = (float)py, X0 = (float)x0;
-            float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
-            float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
-            float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
-            for (uint32_t x = x0; x < px; x++) {
-                w0 = w0 + R.dx[0];
-                w1 = w1 + R.dx[1];
-                w2 = w2 + R.dx[2];
-            }
-            float z;
-            if (frag_z(p, R, w0, w1, w2, z)) update_key(sh.key + tid, mode, p.depth_test, z, D0, B.prim);
-        }
-        __syncthreads();
+        for (int i = 0; i < 6; i++) R.vby[i] = T.vby[i];
+        R.xmin = T.xmin, R.xmax = T.xmax, R.ymin = T.ymin, R.ymax = T.ymax, R.flags = T.flags;
+        const uint32_t ry0 = R.ymin > g.Y0 ? R.ymin : g.Y0;
+        raster_row(p, R, T.prim, ry0 + (k - base), g, mode, sh.key, sh.d0s, sh.band);
     }
+    __syncthreads();  // the next batch reuses sh.tri and sh.rows_incl
 }
 
 // Range [beg, end) of this XCD's tiles: the hardware deals workgroups
@@ -755,7 +747,6 @@ __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
         if (inside && uses_depth) D0 = v.clear_depth_en ? v.clear_depth : v.depth[pidx];
         sh.key[tid] = init_key(mode, D0);
         sh.d0s[tid] = D0;
-        if (tid == 0u) sh.nbig[0] = sh.nbig[1] = 0u;
         if (tid < g.TH) {
             uint32_t by0, by1;
             band_of(p, v, g.Y0 + tid, by0, by1);
@@ -773,22 +764,17 @@ __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
         // this tile's bin, then (only when it overflowed) its entries of the overflow list
         const uint32_t nb = cnt < cap ? cnt : cap;
         const uint32_t nscan = nb + (cnt > cap ? novf : 0u);
-        // nbig[par] counts this batch's large triangles; the other counter is
-        // cleared during the batch (nobody reads it then) for the next one
-        uint32_t par = 0;
         for (uint32_t b = 0; b < nscan; b += 256u) {
             const uint32_t i = b + tid;
-            if (tid == 0u) sh.nbig[par ^ 1u] = 0u;
+            bool valid = false;
+            uint32_t prim = 0;
             if (i < nb) {
-                bin_entry(p, v, sh, bin[i], g, mode, par);
+                valid = true, prim = bin[i];
             } else if (i < nscan) {
                 const uint2 e = p.ovf[i - nb];
-                if (e.x == tile) bin_entry(p, v, sh, e.y, g, mode, par);
+                valid = e.x == tile, prim = e.y;
             }
-            __syncthreads();
-            const uint32_t nbig = sh.nbig[par];
-            if (nbig) big_pass(p, v, sh, nbig, tid, px, py, inside, sx, mode, D0);
-            par ^= 1u;
+            raster_batch(p, v, sh, tid, valid, prim, g, mode);
         }
         if (inside) {
             const uint64_t kk = sh.key[tid];
