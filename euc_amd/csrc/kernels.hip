@@ -618,6 +618,7 @@ EUCHIP_DEV void raster_row(const FrameParams &p, const TriRec &R, uint32_t prim,
 }
 
 constexpr uint32_t kMaxTileH = 32;
+constexpr uint32_t kBatch = 128;  // bin entries staged per batch (at most the workgroup size)
 
 // Setup of one bin entry of the current batch, kept in LDS (24 words).
 struct TriLds {
@@ -628,7 +629,7 @@ struct TriLds {
 struct TileShared {
     uint64_t key[256];              // winner key per pixel
     float d0s[256];                 // depth before the pass (the test's reference)
-    TriLds tri[256];                // the batch's triangles
+    TriLds tri[kBatch];             // the batch's triangles
     uint32_t rows_incl[256];        // inclusive prefix sum of their row counts in the tile
     uint32_t wave_rows[4];          // row counts per wave (prefix-sum carry)
     uint32_t band[2 * kMaxTileH];   // [by0, by1) of the render_par band of each row
@@ -679,9 +680,9 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
     const uint32_t incl = block_inclusive_sum(sh, tid, rows);
     sh.rows_incl[tid] = incl;
     __syncthreads();
-    const uint32_t total = sh.rows_incl[255];
+    const uint32_t total = sh.rows_incl[255];  // lanes past the batch add nothing
     for (uint32_t k = tid; k < total; k += 256u) {
-        uint32_t lo = 0, hi = 255;  // first entry whose inclusive sum exceeds k
+        uint32_t lo = 0, hi = kBatch - 1u;  // first entry whose inclusive sum exceeds k
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
             if (sh.rows_incl[mid] > k) hi = mid;
@@ -711,8 +712,15 @@ EUCHIP_DEV void xcd_range(uint32_t ntiles, uint32_t xcd, uint32_t &beg, uint32_t
     end = beg + q + (xcd < r ? 1u : 0u);
 }
 
+// Waves per SIMD the raster pass is compiled for: latency hiding pays for a few
+// spilled registers with the light shaders; the shadow shaders would spill too
+// much and keep the default.
+constexpr uint32_t raster_waves(uint32_t sh) {
+    return (sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 1u : sh == TEXTURED ? 6u : 7u;
+}
+
 template <uint32_t SH, bool BATCH>
-__global__ __launch_bounds__(256) void k_raster(FrameParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(SH)))) void k_raster(FrameParams p) {
     __shared__ TileShared sh;
     if (p.ctr->flags & FLAG_BIN_OVERFLOW) return;
     const uint32_t tid = threadIdx.x;
@@ -764,11 +772,12 @@ __global__ __launch_bounds__(256) void k_raster(FrameParams p) {
         // this tile's bin, then (only when it overflowed) its entries of the overflow list
         const uint32_t nb = cnt < cap ? cnt : cap;
         const uint32_t nscan = nb + (cnt > cap ? novf : 0u);
-        for (uint32_t b = 0; b < nscan; b += 256u) {
+        for (uint32_t b = 0; b < nscan; b += kBatch) {
             const uint32_t i = b + tid;
             bool valid = false;
             uint32_t prim = 0;
-            if (i < nb) {
+            if (tid >= kBatch) {
+            } else if (i < nb) {
                 valid = true, prim = bin[i];
             } else if (i < nscan) {
                 const uint2 e = p.ovf[i - nb];
