@@ -227,6 +227,18 @@ EUCHIP_DEV const View &view_of(const FrameParams &p, uint32_t r) {
     else return p.v0;
 }
 
+// Stores that bypass the caches on their way to memory: setup records (read
+// back by another launch, usually on another XCD) and the targets (not read
+// again by this frame).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+EUCHIP_DEV void store_stream(uint4 *dst, const uint4 &v) {
+    u32x4 x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<u32x4 *>(dst));
+}
+EUCHIP_DEV void store_stream(uint32_t *dst, uint32_t v) { __builtin_nontemporal_store(v, dst); }
+EUCHIP_DEV void store_stream(float *dst, float v) { __builtin_nontemporal_store(v, dst); }
+
 // Setup records (layout in internal.hpp).
 EUCHIP_DEV void pack_rec(const TriRec &R, uint4 q0[kRec0Quads], uint4 q1[kRec1Quads]) {
     q0[0] = make_uint4(__float_as_uint(R.o[0]), __float_as_uint(R.o[1]), __float_as_uint(R.o[2]), __float_as_uint(R.dx[0]));
@@ -387,12 +399,12 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
 #pragma unroll
         for (uint32_t k = 0; k < kRec0Quads; k++) {
             const uint32_t q = k * 256u + tid;
-            if (sm.rec.kept[q / kRec0Quads]) p.rec0[base * kRec0Quads + q] = sm.rec.q0[q];
+            if (sm.rec.kept[q / kRec0Quads]) store_stream(&p.rec0[base * kRec0Quads + q], sm.rec.q0[q]);
         }
 #pragma unroll
         for (uint32_t k = 0; k < kRec1Quads; k++) {
             const uint32_t q = k * 256u + tid;
-            if (sm.rec.kept[q / kRec1Quads]) p.rec1[base * kRec1Quads + q] = sm.rec.q1[q];
+            if (sm.rec.kept[q / kRec1Quads]) store_stream(&p.rec1[base * kRec1Quads + q], sm.rec.q1[q]);
         }
     }
     __syncthreads();  // the binning tables reuse the record staging
@@ -805,15 +817,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
                     TriRec R;
                     load_rec0(p, v.tri_base + prim, R);
                     tri_sids(v, prim, R.flags, R.sid);
-                    v.pixel[pidx] = shade_pixel<SH>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
-                                                    v.clear_color_en ? v.clear_color : 0u);
+                    store_stream(&v.pixel[pidx], shade_pixel<SH>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
+                                                    v.clear_color_en ? v.clear_color : 0u));
                 } else if (v.clear_color_en) {
-                    v.pixel[pidx] = v.clear_color;
+                    store_stream(&v.pixel[pidx], v.clear_color);
                 }
             }
             if (v.depth) {
-                if (has && p.depth_write) v.depth[pidx] = z;
-                else if (v.clear_depth_en) v.depth[pidx] = v.clear_depth;
+                if (has && p.depth_write) store_stream(&v.depth[pidx], z);
+                else if (v.clear_depth_en) store_stream(&v.depth[pidx], v.clear_depth);
             }
         }
         if (tid == 0u) sh.next = t0 + per_xcd + atomicAdd(next, 1u);
