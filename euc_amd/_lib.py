@@ -61,7 +61,8 @@ class ServeRequestC(C.Structure):
 
 
 class CompletionC(C.Structure):
-    _fields_ = [("ticket", C.c_uint64), ("status", C.c_int), ("pixel", C.c_void_p), ("depth", C.c_void_p)]
+    _fields_ = [("ticket", C.c_uint64), ("status", C.c_int), ("pixel", C.c_void_p), ("depth", C.c_void_p),
+                ("done_ns", C.c_uint64)]
 
 
 class ServerStatsC(C.Structure):
@@ -119,8 +120,11 @@ SIGNATURES = {
                                        C.POINTER(C.c_void_p)]),
     "euchip_server_destroy": (C.c_int, [C.c_void_p]),
     "euchip_server_submit": (C.c_int, [C.c_void_p, C.POINTER(ServeRequestC), C.POINTER(C.c_uint64)]),
+    "euchip_server_submit_n": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "euchip_server_poll": (C.c_int, [C.c_void_p, C.POINTER(CompletionC), C.c_uint32, C.POINTER(C.c_uint32)]),
     "euchip_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "euchip_server_release_n": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "euchip_clock_ns": (C.c_uint64, []),
     "euchip_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStatsC)]),
 }
 

@@ -104,9 +104,10 @@ struct euchip_batch {
     std::vector<uint32_t> caps[3];      // bin slots per tile, per tile width (sx - 3) and active request
     bool tables_dirty = true;
     // device tables and scratch
-    DevArray<View> views;
-    DevArray<uint2> blk;
-    DevArray<uint32_t> tile_req;
+    DevArray<uint4> tables;             // views, workgroup map and tile map, one block
+    uint8_t *stage = nullptr;           // pinned staging of the tables
+    size_t stage_bytes = 0;
+    bool counts_dirty = true;           // tile counters need a reset before the next launch
     DevArray<uint4> rec0, rec1;
     DevArray<float> vary;
     DevArray<uint32_t> tile_count, bins, ctrbuf;  // ctrbuf: Counters, then req_max per active request
@@ -471,11 +472,15 @@ uint32_t pow2_at_least(uint64_t n) {
     return c;
 }
 
-// First guess of a request's bin slots per tile: twice its triangles spread
-// evenly over its tiles, at least 16, at most all of them.
+// First guess of a request's bin slots per tile: eight times its triangles
+// spread evenly over its tiles (an object covers a fraction of the target, and
+// entries past the capacity go to the slow overflow list), at least 16, at most
+// all of them, and no more than 8 Mi slots for the request.
 uint32_t initial_cap(uint64_t n_tris, uint64_t ntiles) {
-    const uint64_t even = ntiles ? (2 * n_tris + ntiles - 1) / ntiles : n_tris;
-    return pow2_at_least(std::min<uint64_t>(std::max<uint64_t>(even, 16), std::max<uint64_t>(n_tris, 16)));
+    const uint64_t even = ntiles ? (8 * n_tris + ntiles - 1) / ntiles : n_tris;
+    uint32_t cap = pow2_at_least(std::min<uint64_t>(std::max<uint64_t>(even, 16), std::max<uint64_t>(n_tris, 16)));
+    while (cap > 16 && (uint64_t)cap * ntiles > ((uint64_t)8 << 20)) cap >>= 1;
+    return cap;
 }
 
 void drop_graph(euchip_batch *b) {
@@ -492,17 +497,45 @@ euchip_status build_tables(euchip_batch *b) {
     const uint32_t sx = b->tile_sx;
     std::vector<uint32_t> &caps = b->caps[sx - 3];
     const size_t na = b->active.size();
-    std::vector<View> hv(na);
-    std::vector<uint2> hblk;
-    std::vector<uint32_t> htile;
-    uint64_t tiles = 0, bins = 0, sv = 0, tris = 0;
+    if (caps.size() < na) caps.resize(na, 0);
+    // sizes first: the tables are written straight into pinned staging memory
+    uint64_t tiles = 0, nblk = 0, bins = 0, sv = 0, tris = 0;
     for (size_t k = 0; k < na; k++) {
         const Plan &P = b->plans[b->active[k]];
         View v = P.v;
         tile_grid(v, sx);
         const uint64_t nt = (uint64_t)v.ntx * v.nty;
-        if (caps.size() < na) caps.resize(na, 0);
         if (caps[k] == 0) caps[k] = initial_cap(P.n_tris, nt);
+        tiles += nt;
+        nblk += (std::max<uint64_t>(P.n_sv, P.n_tris) + 255) / 256;
+        bins += nt * caps[k];
+        sv += P.n_sv;
+        tris += P.n_tris;
+    }
+    if (tiles >= (1ull << 32) || sv >= (1ull << 32) || tris >= (1ull << 32) || nblk >= (1ull << 31))
+        return fail(ctx, EUCHIP_INVALID_ARG, "batch too large for 32-bit tile / vertex ids");
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t off_blk = align(na * sizeof(View));
+    const size_t off_tile = off_blk + align(nblk * sizeof(uint2));
+    const size_t table_bytes = std::max<size_t>(off_tile + tiles * 4, 16);
+    if (b->stage_bytes < table_bytes) {
+        if (b->stage) (void)hipHostFree(b->stage);
+        b->stage = nullptr;
+        b->stage_bytes = 0;
+        const size_t want = table_bytes + table_bytes / 2;
+        HIP_TRY(ctx, hipHostMalloc((void **)&b->stage, want, hipHostMallocDefault));
+        b->stage_bytes = want;
+    }
+    View *hv = reinterpret_cast<View *>(b->stage);
+    uint2 *hblk = reinterpret_cast<uint2 *>(b->stage + off_blk);
+    uint32_t *htile = reinterpret_cast<uint32_t *>(b->stage + off_tile);
+    tiles = bins = sv = tris = 0;
+    uint64_t blk = 0;
+    for (size_t k = 0; k < na; k++) {
+        const Plan &P = b->plans[b->active[k]];
+        View v = P.v;
+        tile_grid(v, sx);
+        const uint64_t nt = (uint64_t)v.ntx * v.nty;
         v.tile_base = (uint32_t)tiles;
         v.sv_base = (uint32_t)sv;
         v.tri_base = (uint32_t)tris;
@@ -510,24 +543,22 @@ euchip_status build_tables(euchip_batch *b) {
         v.bin_base = bins;
         hv[k] = v;
         const uint64_t nb = (std::max<uint64_t>(P.n_sv, P.n_tris) + 255) / 256;
-        for (uint64_t j = 0; j < nb; j++) hblk.push_back(make_uint2((uint32_t)k, (uint32_t)j));
-        htile.insert(htile.end(), nt, (uint32_t)k);
+        for (uint64_t j = 0; j < nb; j++) hblk[blk++] = make_uint2((uint32_t)k, (uint32_t)j);
+        std::fill(htile + tiles, htile + tiles + nt, (uint32_t)k);
         tiles += nt;
         bins += nt * caps[k];
         sv += P.n_sv;
         tris += P.n_tris;
-        if (tiles >= (1ull << 32) || sv >= (1ull << 32) || tris >= (1ull << 32) || hblk.size() >= (1ull << 31))
-            return fail(ctx, EUCHIP_INVALID_ARG, "batch too large for 32-bit tile / vertex ids");
     }
     euchip_status s;
     const uint32_t vdc = vd_count(b->desc.shader);
-    if ((s = ensure(ctx, b->views, std::max<size_t>(na, 1), "batch views", b->stream))) return s;
-    if ((s = ensure(ctx, b->blk, std::max<size_t>(hblk.size(), 1), "batch blocks", b->stream))) return s;
-    if ((s = ensure(ctx, b->tile_req, std::max<size_t>(tiles, 1), "batch tiles", b->stream))) return s;
+    if ((s = ensure(ctx, b->tables, (table_bytes + 15) / 16, "batch tables", b->stream))) return s;
     if ((s = ensure(ctx, b->rec0, std::max<size_t>(tris * kRec0Quads, 1), "batch setup records", b->stream))) return s;
     if ((s = ensure(ctx, b->rec1, std::max<size_t>(tris * kRec1Quads, 1), "batch setup records", b->stream))) return s;
     if ((s = ensure(ctx, b->vary, std::max<size_t>(sv * vdc, 1), "batch vertex data", b->stream))) return s;
+    const uint32_t *old_counts = b->tile_count.ptr;
     if ((s = ensure(ctx, b->tile_count, std::max<size_t>(tiles, 1), "batch tile counts", b->stream))) return s;
+    if (b->tile_count.ptr != old_counts) b->counts_dirty = true;
     if ((s = ensure(ctx, b->bins, std::max<size_t>(bins, 1), "batch bins", b->stream))) return s;
     if ((s = ensure(ctx, b->ovf, std::max<size_t>((size_t)1 << 20, tris), "batch overflow list", b->stream))) return s;
     const size_t words = sizeof(Counters) / 4 + na;
@@ -540,11 +571,19 @@ euchip_status build_tables(euchip_batch *b) {
         b->host_ctr_words = words;
     }
     memset(b->host_ctr, 0, words * 4);
-    HIP_TRY(ctx, hipMemcpyAsync(b->views.ptr, hv.data(), na * sizeof(View), hipMemcpyHostToDevice, b->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(b->blk.ptr, hblk.data(), hblk.size() * sizeof(uint2), hipMemcpyHostToDevice, b->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(b->tile_req.ptr, htile.data(), htile.size() * 4, hipMemcpyHostToDevice, b->stream));
-    HIP_TRY(ctx, hipMemsetAsync(b->tile_count.ptr, 0, b->tile_count.cap * 4, b->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(b->stream));  // the host tables are freed on return
+    // One copy from pinned memory; the staging area is rewritten only when the
+    // batch's stream is idle again (set, set_tile_width and wait synchronise it).
+    HIP_TRY(ctx, hipMemcpyAsync(b->tables.ptr, b->stage, table_bytes, hipMemcpyHostToDevice, b->stream));
+    // Tile counters are zero after every completed raster pass; they need a
+    // reset only when new or after a pass that stopped early.
+    if (b->counts_dirty) {
+        HIP_TRY(ctx, launch_fill_u32(b->tile_count.ptr, 0u, b->tile_count.cap, b->stream));
+        b->counts_dirty = false;
+    }
+    const uint8_t *dev = reinterpret_cast<const uint8_t *>(b->tables.ptr);
+    const View *d_views = reinterpret_cast<const View *>(dev);
+    const uint2 *d_blk = reinterpret_cast<const uint2 *>(dev + off_blk);
+    const uint32_t *d_tile = reinterpret_cast<const uint32_t *>(dev + off_tile);
 
     FrameParams &p = b->p;
     memset(&p, 0, sizeof(p));
@@ -562,10 +601,10 @@ euchip_status build_tables(euchip_batch *b) {
     p.ntiles = (uint32_t)tiles;
     p.tile_sx = sx;
     p.n_views = (uint32_t)na;
-    p.views = b->views.ptr;
-    p.blk = b->blk.ptr;
-    p.tile_req = b->tile_req.ptr;
-    b->setup_blocks = (uint32_t)hblk.size();
+    p.views = d_views;
+    p.blk = d_blk;
+    p.tile_req = d_tile;
+    b->setup_blocks = (uint32_t)nblk;
     b->tiles_total = tiles;
     b->bins_total = bins;
     b->sv_total = sv;
@@ -654,7 +693,8 @@ euchip_status wait_batch(euchip_batch *b) {
         if (attempt >= 3) return fail(ctx, EUCHIP_OVER_BUDGET, "batch bin storage overflow");
         euchip_status s;
         if ((s = ensure(ctx, b->ovf, b->ovf.cap * 4, "batch overflow list", b->stream))) return s;
-        b->tables_dirty = true;  // rebuild also zeroes the tile counters the aborted raster left
+        b->tables_dirty = true;
+        b->counts_dirty = true;  // the stopped raster pass left its tile counters
         if ((s = launch_batch(b, false))) return s;
     }
 }
@@ -662,9 +702,10 @@ euchip_status wait_batch(euchip_batch *b) {
 void free_batch_memory(euchip_batch *b) {
     euchip_ctx *ctx = b->ctx;
     drop_graph(b);
-    release(ctx, b->views);
-    release(ctx, b->blk);
-    release(ctx, b->tile_req);
+    release(ctx, b->tables);
+    if (b->stage) (void)hipHostFree(b->stage);
+    b->stage = nullptr;
+    b->stage_bytes = 0;
     release(ctx, b->rec0);
     release(ctx, b->rec1);
     release(ctx, b->vary);
@@ -1087,7 +1128,7 @@ void *euchip_batch_stream(euchip_batch *b) { return b ? (void *)b->stream : null
 
 size_t euchip_batch_scratch_bytes(const euchip_batch *b) {
     if (!b) return 0;
-    return b->views.cap * sizeof(View) + b->blk.cap * sizeof(uint2) + b->tile_req.cap * 4 + b->rec0.cap * 16 +
+    return b->tables.cap * 16 + b->rec0.cap * 16 +
            b->rec1.cap * 16 +
            b->vary.cap * 4 + b->tile_count.cap * 4 + b->bins.cap * 4 + b->ctrbuf.cap * 4 + b->ovf.cap * 8;
 }

@@ -7,7 +7,9 @@
 // and launches it on the slot's stream.  With two or more slots a batch
 // admitted while another one runs overlaps with it on the device.
 #include <algorithm>
+#include <cmath>
 #include <array>
+#include <chrono>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -24,9 +26,13 @@ size_t batch_fixed_bytes();
 
 namespace {
 
+// A request's targets: owning allocations of a size class (so freed targets
+// serve later requests of other sizes), and views of the request's size.
 struct Target {
-    euchip_buffer *pixel = nullptr, *depth = nullptr;
+    euchip_buffer *pix_store = nullptr, *dep_store = nullptr;  // class_texels(cls) texels each
+    euchip_buffer *pixel = nullptr, *depth = nullptr;          // w x h views of the stores
     uint32_t w = 0, h = 0;
+    uint32_t cls = 0;
 };
 
 struct Queued {
@@ -44,7 +50,17 @@ struct Slot {
     std::vector<Target> targets;
 };
 
-size_t target_bytes(uint32_t w, uint32_t h) { return (size_t)w * h * 8; }
+// Size classes of target storage: four per doubling, from 256 texels.
+size_t class_texels(uint32_t c) { return (size_t)std::ceil(256.0 * std::exp2(c / 4.0)); }
+
+uint32_t class_of(uint32_t w, uint32_t h) {
+    const size_t n = (size_t)w * h;
+    uint32_t c = 0;
+    while (class_texels(c) < n) c++;
+    return c;
+}
+
+size_t target_bytes(uint32_t w, uint32_t h) { return class_texels(class_of(w, h)) * 8; }
 
 }  // namespace
 
@@ -58,27 +74,55 @@ struct euchip_server {
     std::deque<Queued> queue;
     std::deque<euchip_completion> done;
     std::map<uint64_t, Target> out;  // targets of completed requests until released
-    std::multimap<std::pair<uint32_t, uint32_t>, Target> pool;
+    std::multimap<uint32_t, Target> pool;  // free target storage by size class
     uint64_t next_ticket = 1;
     euchip_server_stats st{};
 };
 
 namespace {
 
-void destroy_target(Target &t) {
-    if (t.pixel) euchip_buffer_destroy(t.pixel);
+void drop_views(Target &t) {
+    if (t.pixel) euchip_buffer_destroy(t.pixel);  // views own nothing
     if (t.depth) euchip_buffer_destroy(t.depth);
     t.pixel = t.depth = nullptr;
 }
 
-void to_pool(euchip_server *s, Target t) { s->pool.emplace(std::make_pair(t.w, t.h), t); }
+void destroy_target(Target &t) {
+    drop_views(t);
+    if (t.pix_store) euchip_buffer_destroy(t.pix_store);
+    if (t.dep_store) euchip_buffer_destroy(t.dep_store);
+    t.pix_store = t.dep_store = nullptr;
+}
+
+void to_pool(euchip_server *s, Target t) {
+    drop_views(t);
+    if (t.pix_store && t.dep_store) s->pool.emplace(t.cls, t);
+    else destroy_target(t);
+}
 
 bool take_pooled(euchip_server *s, uint32_t w, uint32_t h, Target &t) {
-    auto it = s->pool.find(std::make_pair(w, h));
+    auto it = s->pool.find(class_of(w, h));
     if (it == s->pool.end()) return false;
     t = it->second;
     s->pool.erase(it);
     return true;
+}
+
+// Storage of a fresh target of class t.cls (accounted against the budget).
+euchip_status alloc_store(euchip_server *s, Target &t) {
+    const uint32_t n = (uint32_t)class_texels(t.cls);
+    euchip_status st = euchip_buffer_create(s->ctx, EUCHIP_FORMAT_RGBA8, n, 1, &t.pix_store);
+    if (!st) st = euchip_buffer_create(s->ctx, EUCHIP_FORMAT_F32, n, 1, &t.dep_store);
+    return st;
+}
+
+// The request-sized views over a target's storage.
+euchip_status make_views(euchip_server *s, Target &t) {
+    euchip_status st = euchip_buffer_wrap(s->ctx, EUCHIP_FORMAT_RGBA8, t.w, t.h, euchip_buffer_device_ptr(t.pix_store),
+                                          &t.pixel);
+    if (!st)
+        st = euchip_buffer_wrap(s->ctx, EUCHIP_FORMAT_F32, t.w, t.h, euchip_buffer_device_ptr(t.dep_store), &t.depth);
+    return st;
 }
 
 // Free pooled targets until `bytes` more fit the budget (or the pool is empty).
@@ -100,9 +144,16 @@ void note_peak(euchip_server *s) {
     s->st.peak_bytes = std::max(s->st.peak_bytes, s->st.bytes_in_use);
 }
 
+uint64_t now_ns() {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
 void complete(euchip_server *s, uint64_t ticket, euchip_status st, Target t) {
     euchip_completion c{};
     c.ticket = ticket;
+    c.done_ns = now_ns();
     c.status = st;
     c.pixel = t.pixel;
     c.depth = t.depth;
@@ -186,7 +237,7 @@ bool admit(euchip_server *s, Slot &sl, bool &freed) {
         }
         take.push_back(q);
         s->queue.pop_front();  // q is gone: use take.back()
-        t.w = take.back().w, t.h = take.back().h;
+        t.w = take.back().w, t.h = take.back().h, t.cls = class_of(t.w, t.h);
         tg.push_back(t);
         fresh.push_back(!pooled);
         scratch = sc;
@@ -195,9 +246,8 @@ bool admit(euchip_server *s, Slot &sl, bool &freed) {
     if (take.empty()) return !s->queue.empty() && head_blocked(s, sl, freed, EUCHIP_OVER_BUDGET);
     euchip_status st = EUCHIP_OK;
     for (size_t i = 0; i < tg.size() && !st; i++) {
-        if (!fresh[i]) continue;
-        if (!(st = euchip_buffer_create(s->ctx, EUCHIP_FORMAT_RGBA8, tg[i].w, tg[i].h, &tg[i].pixel)))
-            st = euchip_buffer_create(s->ctx, EUCHIP_FORMAT_F32, tg[i].w, tg[i].h, &tg[i].depth);
+        if (fresh[i]) st = alloc_store(s, tg[i]);
+        if (!st) st = make_views(s, tg[i]);
     }
     std::vector<euchip_request> reqs(take.size());
     if (!st) {
@@ -214,10 +264,7 @@ bool admit(euchip_server *s, Slot &sl, bool &freed) {
     if (!st) st = euchip_batch_launch(sl.batch, 0);
     if (st == EUCHIP_OVER_BUDGET || st == EUCHIP_OOM) {
         // memory ran out after all: hand everything back and admit fewer next time
-        for (Target &t : tg) {
-            if (t.pixel && t.depth) to_pool(s, t);
-            else destroy_target(t);
-        }
+        for (Target &t : tg) to_pool(s, t);
         for (size_t i = take.size(); i-- > 0;) s->queue.push_front(take[i]);
         if (s->batch_cap > 1) {
             s->batch_cap = std::max<uint32_t>(1, s->batch_cap / 2);
@@ -300,6 +347,16 @@ euchip_status euchip_server_submit(euchip_server *s, const euchip_serve_request 
     return EUCHIP_OK;
 }
 
+euchip_status euchip_server_submit_n(euchip_server *s, const euchip_serve_request *reqs, uint32_t n,
+                                     uint64_t *tickets) {
+    if (!s || (n && !reqs)) return EUCHIP_INVALID_ARG;
+    for (uint32_t i = 0; i < n; i++) {
+        const euchip_status st = euchip_server_submit(s, &reqs[i], tickets ? &tickets[i] : nullptr);
+        if (st) return st;
+    }
+    return EUCHIP_OK;
+}
+
 euchip_status euchip_server_poll(euchip_server *s, euchip_completion *out, uint32_t max, uint32_t *n_out) {
     if (!s) return EUCHIP_INVALID_ARG;
     retire(s);
@@ -327,6 +384,16 @@ euchip_status euchip_server_release(euchip_server *s, uint64_t ticket) {
     s->out.erase(it);
     return EUCHIP_OK;
 }
+
+euchip_status euchip_server_release_n(euchip_server *s, const uint64_t *tickets, uint32_t n) {
+    if (!s || (n && !tickets)) return EUCHIP_INVALID_ARG;
+    euchip_status bad = EUCHIP_OK;
+    for (uint32_t i = 0; i < n; i++)
+        if (euchip_server_release(s, tickets[i])) bad = EUCHIP_INVALID_ARG;
+    return bad;
+}
+
+uint64_t euchip_clock_ns(void) { return now_ns(); }
 
 euchip_status euchip_server_get_stats(const euchip_server *s, euchip_server_stats *out) {
     if (!s || !out) return EUCHIP_INVALID_ARG;

@@ -133,38 +133,66 @@ def poisson_arrivals(n: int, rate_hz: float, seed: int) -> np.ndarray:
 def run_poisson(server: Server, items: list, rate_hz: float, seed: int = 1, keep: Optional[Callable] = None,
                 timeout_s: float = 120.0) -> ServeResult:
     """Submit `items` at Poisson arrival times of rate `rate_hz` and poll until all
-    complete.  keep(i) -> bool selects requests whose colour and depth are
-    downloaded into the result; every target is released after completion."""
-    arrivals = poisson_arrivals(len(items), rate_hz, seed)
+    complete.  Requests are submitted in bulk as they arrive (one native call per
+    loop turn) and completion times are the server's own (euchip_clock_ns), so
+    the loop's cost does not grow with the request count.  keep(i) -> bool selects
+    requests whose colour and depth are downloaded into the result; every target
+    is released after completion."""
+    L = _lib.lib()
     n = len(items)
-    ticket_of = {}
-    done_at = np.full(n, np.nan)
+    reqs = (ServeRequestC * max(n, 1))()
+    hold = []
+    for i, it in enumerate(items):
+        u = np.ascontiguousarray(it.uniforms, dtype=np.float32)
+        assert u.size == N_UNIFORMS
+        hold.append(u)
+        reqs[i] = ServeRequestC(mesh=it.mesh.handle, texture=it.texture.handle if it.texture else None, shadow=None,
+                                uniforms=u.ctypes.data, width=int(it.width), height=int(it.height))
+    tickets = (C.c_uint64 * max(n, 1))()
+    comp = (CompletionC * 1024)()
+    rel = (C.c_uint64 * 1024)()
+    got = C.c_uint32()
+    rsize = C.sizeof(ServeRequestC)
+    arrivals = poisson_arrivals(n, rate_hz, seed)
+    arr_ns = (arrivals * 1e9).astype(np.int64)
+    done_ns = np.zeros(n, np.int64)
     statuses = [None] * n
     outputs = {}
-    i_next = 0
-    t0 = time.perf_counter()
-    while True:
-        now = time.perf_counter() - t0
-        while i_next < n and arrivals[i_next] <= now:
-            it = items[i_next]
-            ticket_of[server.submit(it.mesh, it.uniforms, it.width, it.height, it.texture)] = i_next
-            i_next += 1
-        for c in server.poll():
-            i = ticket_of.pop(c.ticket)
-            done_at[i] = time.perf_counter() - t0
+    base = None
+    i_next, remaining = 0, n
+    t0 = L.euchip_clock_ns()
+    while remaining:
+        now = L.euchip_clock_ns() - t0
+        j = int(np.searchsorted(arr_ns, now, side="right"))
+        if j > i_next:
+            _check(server.ctx.handle, L.euchip_server_submit_n(server.handle, C.byref(reqs, i_next * rsize), j - i_next,
+                                                               C.byref(tickets, i_next * 8)))
+            if base is None:
+                base = tickets[0]
+            i_next = j
+        _check(server.ctx.handle, L.euchip_server_poll(server.handle, comp, len(comp), C.byref(got)))
+        nrel = 0
+        for c in comp[:got.value]:
+            i = c.ticket - base
+            server.sizes.pop(c.ticket, None)
+            done_ns[i] = c.done_ns - t0
             statuses[i] = c.status
-            if c.status == 0 and keep is not None and keep(i):
-                outputs[i] = (c.pixel.download(), c.depth.download())
+            remaining -= 1
             if c.status == 0:
-                server.release(c.ticket)
-        if i_next == n and not ticket_of:
-            break
-        if time.perf_counter() - t0 > timeout_s:
-            raise TimeoutError(f"serving loop: {len(ticket_of)} requests still pending after {timeout_s} s")
-        if i_next < n and not ticket_of:
-            wait = arrivals[i_next] - (time.perf_counter() - t0)
+                if keep is not None and keep(i):
+                    w, h = items[i].width, items[i].height
+                    outputs[i] = (BorrowedBuffer(server.ctx, c.pixel, FORMAT_RGBA8, w, h).download(),
+                                  BorrowedBuffer(server.ctx, c.depth, FORMAT_F32, w, h).download())
+                rel[nrel] = c.ticket
+                nrel += 1
+        if nrel:
+            _check(server.ctx.handle, L.euchip_server_release_n(server.handle, rel, nrel))
+        if L.euchip_clock_ns() - t0 > timeout_s * 1e9:
+            raise TimeoutError(f"serving loop: {remaining} requests still pending after {timeout_s} s")
+        if i_next < n and remaining == n - i_next:  # nothing in flight: wait for the next arrival
+            wait = (arr_ns[i_next] - (L.euchip_clock_ns() - t0)) * 1e-9
             if wait > 0.0005:
                 time.sleep(wait - 0.0002)
-    lat = done_at - arrivals
-    elapsed = float(np.nanmax(done_at) - arrivals[0]) if n else 0.0
+    lat = (done_ns - arr_ns) * 1e-9
+    elapsed = float((done_ns.max() - arr_ns[0]) * 1e-9) if n else 0.0
     return ServeResult(lat, elapsed, statuses, server.stats(), outputs)

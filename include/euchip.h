@@ -249,6 +249,7 @@ typedef struct euchip_completion {
     euchip_status status;          /* EUCHIP_OVER_BUDGET: the request alone does not fit the budget */
     euchip_buffer *pixel;          /* valid until euchip_server_release(ticket) */
     euchip_buffer *depth;
+    uint64_t done_ns;              /* euchip_clock_ns() when the server saw the request finish */
 } euchip_completion;
 
 typedef struct euchip_server_stats {
@@ -263,11 +264,17 @@ euchip_status euchip_server_create(euchip_ctx *ctx, const euchip_pipeline_desc *
                                    uint32_t max_batch, uint32_t slots, euchip_server **out);
 euchip_status euchip_server_destroy(euchip_server *srv);
 euchip_status euchip_server_submit(euchip_server *srv, const euchip_serve_request *req, uint64_t *ticket);
+/* Submit n requests in order; tickets (may be NULL) receives their tickets. */
+euchip_status euchip_server_submit_n(euchip_server *srv, const euchip_serve_request *reqs, uint32_t n,
+                                     uint64_t *tickets);
 /* Retire finished batches, admit and launch queued requests; copies up to max
  * completions (oldest first) to out and their number to *n_out. */
 euchip_status euchip_server_poll(euchip_server *srv, euchip_completion *out, uint32_t max, uint32_t *n_out);
 /* Hand a completed request's targets back to the server's pool. */
 euchip_status euchip_server_release(euchip_server *srv, uint64_t ticket);
+euchip_status euchip_server_release_n(euchip_server *srv, const uint64_t *tickets, uint32_t n);
+/* The host clock of completion times (monotonic, nanoseconds). */
+uint64_t euchip_clock_ns(void);
 euchip_status euchip_server_get_stats(const euchip_server *srv, euchip_server_stats *out);
 
 #ifdef __cplusplus
