@@ -51,8 +51,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--shader", default="phong", choices=["phong", "gouraud"])
     ap.add_argument("--workload", default="torus", choices=["torus", "icons", "serving", "heightfield"])
-    ap.add_argument("--rate", type=float, default=4000.0, help="Poisson arrival rate of the serving workload (req/s)")
-    ap.add_argument("--requests", type=int, default=3000, help="requests of the serving workload")
+    ap.add_argument("--rate", type=float, default=40000.0, help="Poisson arrival rate of the serving workload (req/s)")
+    ap.add_argument("--requests", type=int, default=20000, help="requests of the serving workload")
     ap.add_argument("--budget-gib", type=float, default=16.0, help="device-memory budget of the serving workload")
     ap.add_argument("--icons", type=int, default=4096, help="requests per batch of the icons workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
