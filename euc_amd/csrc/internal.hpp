@@ -55,10 +55,17 @@ enum FrameFlags : uint32_t { FLAG_BIN_OVERFLOW = 1u };
 struct Counters {
     uint32_t ovf[2][32];
     uint32_t next[2][8][32];  // per frame parity and XCD: next tile of the dynamic tile loop
+    uint32_t heavy[2][32];    // per frame parity: entries of the heavy-tile list
     uint32_t flags;      // FrameFlags, sticky until the host clears them
     uint32_t max_count;  // largest per-tile triangle count seen by a one-request frame
     uint32_t pad[30];
 };
+
+// Tiles whose bin reaches this many entries are listed by setup and rasterised
+// first (the longest tiles start early instead of ending the launch late).
+constexpr uint32_t kHeavyTile = 160;
+// tile_count value of a tile the heavy pass has taken
+constexpr uint32_t kTileTaken = 0xffffffffu;
 
 // One render request: a target pair, a mesh, a uniform block, and where the
 // request's vertices, tiles and bins sit in the launch's scratch memory.
@@ -109,6 +116,8 @@ struct FrameParams {
     uint32_t *req_max;     // per request: largest tile count above its tile_cap (host grows the bins)
     uint2 *ovf;            // (tile, primitive) of ranks >= tile_cap
     uint32_t ovf_cap;
+    uint32_t *heavy;       // tiles with at least kHeavyTile entries (heavy_cap slots)
+    uint32_t heavy_cap;    // 0: no heavy-first pass
     uint32_t frame;        // frame parity selects the overflow counter and tile queue
     uint32_t ntiles;       // tiles of all requests
     uint32_t tile_sx;      // tiles are (1 << tile_sx) x (256 >> tile_sx) pixels, tile_sx in 3..5

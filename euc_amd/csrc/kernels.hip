@@ -214,9 +214,19 @@ EUCHIP_DEV void bin_store(const FrameParams &p, const View &v, uint32_t tile, ui
     }
 }
 
+// A tile whose count went from `old` to `old + n` may have become heavy.
+EUCHIP_DEV void note_heavy(const FrameParams &p, uint32_t tile, uint32_t old, uint32_t n) {
+    if (p.heavy_cap && old < kHeavyTile && old + n >= kHeavyTile) {
+        const uint32_t k = atomicAdd(&p.ctr->heavy[p.frame & 1u][0], 1u);
+        if (k < p.heavy_cap) p.heavy[k] = tile;
+    }
+}
+
 // Bin append of one pair by a global atomic (the fallback of the LDS-grouped path).
 EUCHIP_DEV void bin_direct(const FrameParams &p, const View &v, uint32_t tile, uint32_t prim) {
-    bin_store(p, v, tile, atomicAdd(p.tile_count + tile, 1u), prim);
+    const uint32_t rank = atomicAdd(p.tile_count + tile, 1u);
+    note_heavy(p, tile, rank, 1u);
+    bin_store(p, v, tile, rank, prim);
 }
 
 // The request a workgroup or tile works on: the kernel argument's, or one of
@@ -432,8 +442,13 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
         bin_direct(p, v, tile, prim);  // table or pair list full
     });
     __syncthreads();
-    for (uint32_t i = tid; i < kHash; i += 256u)
-        if (sm.bin.hcnt[i]) sm.bin.hbase[i] = atomicAdd(p.tile_count + (sm.bin.hkey[i] - 1u), sm.bin.hcnt[i]);
+    for (uint32_t i = tid; i < kHash; i += 256u) {
+        if (sm.bin.hcnt[i]) {
+            const uint32_t tile = sm.bin.hkey[i] - 1u;
+            sm.bin.hbase[i] = atomicAdd(p.tile_count + tile, sm.bin.hcnt[i]);
+            note_heavy(p, tile, sm.bin.hbase[i], sm.bin.hcnt[i]);
+        }
+    }
     __syncthreads();
     const uint32_t np = npl < kPairs ? npl : kPairs;
     for (uint32_t k = tid; k < np; k += 256u) {
@@ -646,6 +661,7 @@ struct TileShared {
     uint32_t wave_rows[4];          // row counts per wave (prefix-sum carry)
     uint32_t band[2 * kMaxTileH];   // [by0, by1) of the render_par band of each row
     uint32_t next;                  // tile taken from the dynamic queue
+    uint32_t cnt;                   // entries of the current tile (kTileTaken: skip it)
 };
 
 // Rows of bin entry `prim` inside the tile, its setup kept in sh.tri[slot].
@@ -751,7 +767,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     const uint32_t per_xcd = gridDim.x >> 3;
     uint32_t *next = &p.ctr->next[p.frame & 1u][xcd][0];
     if (blockIdx.x < 8u && tid == 0u) p.ctr->next[(p.frame + 1u) & 1u][xcd][0] = 0u;  // next frame's queue
-    for (uint32_t tile = t0 + (blockIdx.x >> 3); tile < t1;) {
+    // Workgroup b first takes heavy tile b (if there is one), then its tiles as
+    // above; a tile is claimed by exchanging its count, and the heavy pass leaves
+    // kTileTaken so the regular pass skips it (and zeroes the count for the next frame).
+    // Batches skip this: their tiles are mostly busy and keep their order.
+    bool heavy_pass = false;
+    uint32_t tile = t0 + (blockIdx.x >> 3);
+    if constexpr (!BATCH) {
+        if (blockIdx.x == 0u && tid == 0u) p.ctr->heavy[(p.frame + 1u) & 1u][0] = 0u;
+        uint32_t nh = __builtin_amdgcn_readfirstlane(p.ctr->heavy[p.frame & 1u][0]);
+        nh = nh < p.heavy_cap ? nh : p.heavy_cap;
+        heavy_pass = blockIdx.x < nh;
+        if (heavy_pass) tile = __builtin_amdgcn_readfirstlane(p.heavy[blockIdx.x]);
+    }
+    while (heavy_pass || tile < t1) {
         uint32_t r = 0;
         if constexpr (BATCH) r = __builtin_amdgcn_readfirstlane(p.tile_req[tile]);
         const View &v = view_of<BATCH>(p, r);
@@ -775,58 +804,81 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         }
         const uint32_t cap = v.tile_cap;
         const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
-        const uint32_t cnt = p.tile_count[tile];
-        __syncthreads();
-        if (tid == 0u) {
-            p.tile_count[tile] = 0u;  // counter of the next frame
-            if (cnt > cap) atomicMax(p.req_max + r, cnt);
-        }
-        // this tile's bin, then (only when it overflowed) its entries of the overflow list
-        const uint32_t nb = cnt < cap ? cnt : cap;
-        const uint32_t nscan = nb + (cnt > cap ? novf : 0u);
-        for (uint32_t b = 0; b < nscan; b += kBatch) {
-            const uint32_t i = b + tid;
-            bool valid = false;
-            uint32_t prim = 0;
-            if (tid >= kBatch) {
-            } else if (i < nb) {
-                valid = true, prim = bin[i];
-            } else if (i < nscan) {
-                const uint2 e = p.ovf[i - nb];
-                valid = e.x == tile, prim = e.y;
+        uint32_t cnt;
+        if constexpr (BATCH) {
+            cnt = p.tile_count[tile];
+            __syncthreads();
+            if (tid == 0u) {
+                p.tile_count[tile] = 0u;  // counter of the next frame
+                if (cnt > cap) atomicMax(p.req_max + r, cnt);
             }
-            raster_batch(p, v, sh, tid, valid, prim, g, mode);
-        }
-        if (inside) {
-            const uint64_t kk = sh.key[tid];
-            bool has;
-            uint32_t prim;
-            float z;
-            if (mode == 0u) {
-                has = kk != 0ull;
-                prim = (uint32_t)(kk >> 32) - 1u;
-                z = __uint_as_float((uint32_t)kk);
-            } else {
-                has = ((uint32_t)kk) != 0u;
-                prim = ((uint32_t)kk >> 1) - 1u;
-                const uint32_t o = (uint32_t)(kk >> 32);
-                z = zunord(mode == 1u ? o : ~o, (uint32_t)kk & 1u);
+        } else {
+            if (tid == 0u) {
+                uint32_t c = atomicExch(p.tile_count + tile, heavy_pass ? kTileTaken : 0u);
+                if (heavy_pass && c == 0u) {  // the regular pass took it first
+                    p.tile_count[tile] = 0u;
+                    c = kTileTaken;
+                }
+                sh.cnt = c;
+                if (c != kTileTaken && c > cap) atomicMax(p.req_max + r, c);
             }
-            if (v.pixel) {
-                if (p.pixel_write && has && SH != DEPTH_ONLY) {
-                    TriRec R;
-                    load_rec0(p, v.tri_base + prim, R);
-                    tri_sids(v, prim, R.flags, R.sid);
-                    store_stream(&v.pixel[pidx], shade_pixel<SH>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
-                                                    v.clear_color_en ? v.clear_color : 0u));
-                } else if (v.clear_color_en) {
-                    store_stream(&v.pixel[pidx], v.clear_color);
+            __syncthreads();
+            cnt = sh.cnt;
+        }
+        if (cnt != kTileTaken) {
+            // this tile's bin, then (only when it overflowed) its entries of the overflow list
+            const uint32_t nb = cnt < cap ? cnt : cap;
+            const uint32_t nscan = nb + (cnt > cap ? novf : 0u);
+            for (uint32_t b = 0; b < nscan; b += kBatch) {
+                const uint32_t i = b + tid;
+                bool valid = false;
+                uint32_t prim = 0;
+                if (tid >= kBatch) {
+                } else if (i < nb) {
+                    valid = true, prim = bin[i];
+                } else if (i < nscan) {
+                    const uint2 e = p.ovf[i - nb];
+                    valid = e.x == tile, prim = e.y;
+                }
+                raster_batch(p, v, sh, tid, valid, prim, g, mode);
+            }
+            if (inside) {
+                const uint64_t kk = sh.key[tid];
+                bool has;
+                uint32_t prim;
+                float z;
+                if (mode == 0u) {
+                    has = kk != 0ull;
+                    prim = (uint32_t)(kk >> 32) - 1u;
+                    z = __uint_as_float((uint32_t)kk);
+                } else {
+                    has = ((uint32_t)kk) != 0u;
+                    prim = ((uint32_t)kk >> 1) - 1u;
+                    const uint32_t o = (uint32_t)(kk >> 32);
+                    z = zunord(mode == 1u ? o : ~o, (uint32_t)kk & 1u);
+                }
+                if (v.pixel) {
+                    if (p.pixel_write && has && SH != DEPTH_ONLY) {
+                        TriRec R;
+                        load_rec0(p, v.tri_base + prim, R);
+                        tri_sids(v, prim, R.flags, R.sid);
+                        store_stream(&v.pixel[pidx], shade_pixel<SH>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
+                                                        v.clear_color_en ? v.clear_color : 0u));
+                    } else if (v.clear_color_en) {
+                        store_stream(&v.pixel[pidx], v.clear_color);
+                    }
+                }
+                if (v.depth) {
+                    if (has && p.depth_write) store_stream(&v.depth[pidx], z);
+                    else if (v.clear_depth_en) store_stream(&v.depth[pidx], v.clear_depth);
                 }
             }
-            if (v.depth) {
-                if (has && p.depth_write) store_stream(&v.depth[pidx], z);
-                else if (v.clear_depth_en) store_stream(&v.depth[pidx], v.clear_depth);
-            }
+        }  // cnt != kTileTaken
+        if (heavy_pass) {
+            heavy_pass = false;
+            tile = t0 + (blockIdx.x >> 3);
+            __syncthreads();  // LDS is reused by the next tile
+            continue;
         }
         if (tid == 0u) sh.next = t0 + per_xcd + atomicAdd(next, 1u);
         __syncthreads();  // LDS is reused by the next tile

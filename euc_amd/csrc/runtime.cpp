@@ -54,6 +54,7 @@ struct euchip_ctx {
     DevArray<float> vary;
     DevArray<uint32_t> tile_count, bins;
     DevArray<uint2> ovf;
+    DevArray<uint32_t> heavy;   // heavy-tile list
     uint32_t tile_cap = 128;    // bin slots per tile; raised when a frame needed more
     uint32_t frame = 0;
     uint32_t max_groups = 2048; // resident raster workgroups: CUs x 8
@@ -431,6 +432,7 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     if (fresh_tiles) HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4, ctx->stream));
     if ((s = ensure(ctx, ctx->ovf, std::max<size_t>((size_t)1 << 20, (size_t)P.n_tris), "overflow list", ctx->stream)))
         return s;
+    if ((s = ensure(ctx, ctx->heavy, ntiles, "heavy tiles", ctx->stream))) return s;
 
     for (int attempt = 0;; attempt++) {
         if ((s = ensure(ctx, ctx->bins, ntiles * ctx->tile_cap, "bins", ctx->stream))) return s;
@@ -445,6 +447,8 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.v0.bin_base = 0;
         p.ovf = ctx->ovf.ptr;
         p.ovf_cap = (uint32_t)std::min<size_t>(ctx->ovf.cap, 0xFFFFFFF0ull);
+        p.heavy = ctx->heavy.ptr;
+        p.heavy_cap = (uint32_t)std::min<size_t>(ctx->heavy.cap, 0xFFFFFFF0ull);
         p.frame = ctx->frame++;
         if ((s = timed(ctx, "setup", [&] { return launch_setup(p, ctx->stream, false, 0); }))) return s;
         if ((s = timed(ctx, "raster", [&] { return launch_raster(p, ctx->stream, false, ctx->max_groups); }))) return s;
@@ -597,6 +601,8 @@ euchip_status build_tables(euchip_batch *b) {
     p.bins = b->bins.ptr;
     p.ovf = b->ovf.ptr;
     p.ovf_cap = (uint32_t)std::min<size_t>(b->ovf.cap, 0xFFFFFFF0ull);
+    p.heavy = nullptr;  // batches keep their tile order: most of their tiles would be heavy
+    p.heavy_cap = 0;
     p.frame = 0;
     p.ntiles = (uint32_t)tiles;
     p.tile_sx = sx;
@@ -790,6 +796,7 @@ euchip_status euchip_ctx_destroy(euchip_ctx *ctx) {
     release(ctx, ctx->tile_count);
     release(ctx, ctx->bins);
     release(ctx, ctx->ovf);
+    release(ctx, ctx->heavy);
     (void)hipFree(ctx->counters);
     if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
