@@ -56,16 +56,21 @@ struct Counters {
     uint32_t ovf[2][32];
     uint32_t next[2][8][32];  // per frame parity and XCD: next tile of the dynamic tile loop
     uint32_t heavy[2][32];    // per frame parity: entries of the heavy-tile list
+    uint32_t busy[2][8][32];  // per frame parity and XCD: entries of the busy-tile list
     uint32_t flags;      // FrameFlags, sticky until the host clears them
     uint32_t max_count;  // largest per-tile triangle count seen by a one-request frame
     uint32_t pad[30];
 };
 
+// Bin entries the raster pass stages per step.
+constexpr uint32_t kBatch = 128;
 // Tiles whose bin reaches this many entries are listed by setup and rasterised
 // first (the longest tiles start early instead of ending the launch late).
-constexpr uint32_t kHeavyTile = 160;
-// tile_count value of a tile the heavy pass has taken
-constexpr uint32_t kTileTaken = 0xffffffffu;
+constexpr uint32_t kHeavyTile = 160u;
+// Busy-tile lists of one-request frames: 0 = one per XCD's contiguous tile
+// range, 1 = one list, 2 = blocks of 4 x 4 tiles dealt over the 8 XCDs (each
+// XCD gets a share of every screen region, so dense regions do not load one XCD).
+constexpr uint32_t kBusyMap = 2;
 
 // One render request: a target pair, a mesh, a uniform block, and where the
 // request's vertices, tiles and bins sit in the launch's scratch memory.
@@ -111,13 +116,23 @@ struct FrameParams {
     uint4 *rec1;           // kRec1Quads per triangle of the launch
     float *vary;           // VertexData per shaded vertex
     Counters *ctr;
-    uint32_t *tile_count;  // per tile of the launch, zero between frames
+    uint32_t *tile_count;  // per tile of the launch, zero before setup
     uint32_t *bins;        // bin_base + local tile * tile_cap + rank -> primitive
     uint32_t *req_max;     // per request: largest tile count above its tile_cap (host grows the bins)
     uint2 *ovf;            // (tile, primitive) of ranks >= tile_cap
     uint32_t ovf_cap;
     uint32_t *heavy;       // tiles with at least kHeavyTile entries (heavy_cap slots)
     uint32_t heavy_cap;    // 0: no heavy-first pass
+    // One-request frames: setup lists the tiles it bins into (busy) in 8 lists
+    // of up to ntiles entries (list x at busy + x * ntiles, see kBusyMap); raster
+    // clears the empty tiles itself, takes only busy tiles from its queues,
+    // leaves `tile_count` as it is and zeroes the first `zero_n` counts of
+    // `count_next` (the other half of a two-frame count buffer).  busy ==
+    // nullptr: one tile queue per XCD range, and raster re-zeroes each count it
+    // reads (batches).
+    uint32_t *busy;
+    uint32_t *count_next;
+    uint32_t zero_n;
     uint32_t frame;        // frame parity selects the overflow counter and tile queue
     uint32_t ntiles;       // tiles of all requests
     uint32_t tile_sx;      // tiles are (1 << tile_sx) x (256 >> tile_sx) pixels, tile_sx in 3..5

@@ -214,6 +214,32 @@ EUCHIP_DEV void bin_store(const FrameParams &p, const View &v, uint32_t tile, ui
     }
 }
 
+// Range [beg, end) of this XCD's tiles: the hardware deals workgroups
+// round-robin to the 8 XCDs, so workgroup b runs on XCD b % 8; each XCD gets a
+// contiguous run of tiles so triangles shared by neighbouring tiles stay in
+// that XCD's L2.
+EUCHIP_DEV void xcd_range(uint32_t ntiles, uint32_t xcd, uint32_t &beg, uint32_t &end) {
+    const uint32_t q = ntiles >> 3, r = ntiles & 7u;
+    beg = xcd * q + (xcd < r ? xcd : r);
+    end = beg + q + (xcd < r ? 1u : 0u);
+}
+
+// Busy list of a tile (list x holds up to ntiles entries at p.busy + x * ntiles).
+// kBusyMap 0: the XCD whose contiguous tile range holds the tile; 1: one list
+// for the whole launch; 2: blocks of 4 x 4 tiles dealt over the 8 XCDs.
+EUCHIP_DEV uint32_t busy_list_of(const FrameParams &p, uint32_t tile) {
+    if constexpr (kBusyMap == 1u) {
+        return 0u;
+    } else if constexpr (kBusyMap == 2u) {
+        const uint32_t ty = tile / p.v0.ntx, tx = tile - ty * p.v0.ntx;
+        return ((tx >> 2) + 3u * (ty >> 2)) & 7u;
+    } else {
+        const uint32_t q = p.ntiles >> 3, r = p.ntiles & 7u;
+        const uint32_t big = r * (q + 1u);  // tiles of the ranges one longer
+        return tile < big ? tile / (q + 1u) : r + (tile - big) / q;
+    }
+}
+
 // A tile whose count went from `old` to `old + n` may have become heavy.
 EUCHIP_DEV void note_heavy(const FrameParams &p, uint32_t tile, uint32_t old, uint32_t n) {
     if (p.heavy_cap && old < kHeavyTile && old + n >= kHeavyTile) {
@@ -226,6 +252,11 @@ EUCHIP_DEV void note_heavy(const FrameParams &p, uint32_t tile, uint32_t old, ui
 EUCHIP_DEV void bin_direct(const FrameParams &p, const View &v, uint32_t tile, uint32_t prim) {
     const uint32_t rank = atomicAdd(p.tile_count + tile, 1u);
     note_heavy(p, tile, rank, 1u);
+    if (p.busy && rank == 0u) {
+        const uint32_t x = busy_list_of(p, tile);
+        const uint32_t k = atomicAdd(&p.ctr->busy[p.frame & 1u][x][0], 1u);
+        if (k < p.ntiles) p.busy[(size_t)x * p.ntiles + k] = tile;
+    }
     bin_store(p, v, tile, rank, prim);
 }
 
@@ -360,6 +391,7 @@ template <uint32_t SH, bool BATCH>
 __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
     __shared__ SetupShared sm;
     __shared__ uint32_t npl;
+    __shared__ uint32_t lcnt[9], lbase[9];  // new busy tiles per list (8) and new heavy tiles (8)
     const uint32_t tid = threadIdx.x;
     uint32_t r = 0, j = blockIdx.x;
     if constexpr (BATCH) {
@@ -442,11 +474,43 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
         bin_direct(p, v, tile, prim);  // table or pair list full
     });
     __syncthreads();
+    // One global atomic per distinct tile; tiles that became busy or heavy get a
+    // rank among this workgroup's new ones (kept in hcnt, no longer needed):
+    // bits 0-10 busy rank, 11-13 busy list, 14 busy, 15-25 heavy rank, 26 heavy.
+    if (tid < 9u) lcnt[tid] = 0u;
+    __syncthreads();
     for (uint32_t i = tid; i < kHash; i += 256u) {
-        if (sm.bin.hcnt[i]) {
+        const uint32_t n = sm.bin.hcnt[i];
+        if (n) {
             const uint32_t tile = sm.bin.hkey[i] - 1u;
-            sm.bin.hbase[i] = atomicAdd(p.tile_count + tile, sm.bin.hcnt[i]);
-            note_heavy(p, tile, sm.bin.hbase[i], sm.bin.hcnt[i]);
+            const uint32_t old = atomicAdd(p.tile_count + tile, n);
+            sm.bin.hbase[i] = old;
+            uint32_t code = 0u;
+            if (p.busy && old == 0u) {
+                const uint32_t x = busy_list_of(p, tile);
+                code |= atomicAdd(&lcnt[x], 1u) | (x << 11) | (1u << 14);
+            }
+            if (p.heavy_cap && old < kHeavyTile && old + n >= kHeavyTile) code |= (atomicAdd(&lcnt[8], 1u) << 15) | (1u << 26);
+            sm.bin.hcnt[i] = code;
+        }
+    }
+    __syncthreads();
+    if (tid < 9u && lcnt[tid]) {
+        lbase[tid] = tid < 8u ? atomicAdd(&p.ctr->busy[p.frame & 1u][tid][0], lcnt[tid])
+                              : atomicAdd(&p.ctr->heavy[p.frame & 1u][0], lcnt[tid]);
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < kHash; i += 256u) {
+        const uint32_t code = sm.bin.hcnt[i];
+        if (sm.bin.hkey[i] == 0u || code == 0u) continue;
+        const uint32_t tile = sm.bin.hkey[i] - 1u;
+        if (code & (1u << 14)) {
+            const uint32_t x = (code >> 11) & 7u, k = lbase[x] + (code & 0x7ffu);
+            if (k < p.ntiles) p.busy[(size_t)x * p.ntiles + k] = tile;
+        }
+        if (code & (1u << 26)) {
+            const uint32_t k = lbase[8] + ((code >> 15) & 0x7ffu);
+            if (k < p.heavy_cap) p.heavy[k] = tile;
         }
     }
     __syncthreads();
@@ -645,7 +709,7 @@ EUCHIP_DEV void raster_row(const FrameParams &p, const TriRec &R, uint32_t prim,
 }
 
 constexpr uint32_t kMaxTileH = 32;
-constexpr uint32_t kBatch = 128;  // bin entries staged per batch (at most the workgroup size)
+static_assert(kBatch <= 256u, "a batch is staged one entry per lane");
 
 // Setup of one bin entry of the current batch, kept in LDS (24 words).
 struct TriLds {
@@ -661,7 +725,6 @@ struct TileShared {
     uint32_t wave_rows[4];          // row counts per wave (prefix-sum carry)
     uint32_t band[2 * kMaxTileH];   // [by0, by1) of the render_par band of each row
     uint32_t next;                  // tile taken from the dynamic queue
-    uint32_t cnt;                   // entries of the current tile (kTileTaken: skip it)
 };
 
 // Rows of bin entry `prim` inside the tile, its setup kept in sh.tri[slot].
@@ -730,16 +793,6 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
     __syncthreads();  // the next batch reuses sh.tri and sh.rows_incl
 }
 
-// Range [beg, end) of this XCD's tiles: the hardware deals workgroups
-// round-robin to the 8 XCDs, so workgroup b runs on XCD b % 8; each XCD gets a
-// contiguous run of tiles so triangles shared by neighbouring tiles stay in
-// that XCD's L2.
-EUCHIP_DEV void xcd_range(uint32_t ntiles, uint32_t xcd, uint32_t &beg, uint32_t &end) {
-    const uint32_t q = ntiles >> 3, r = ntiles & 7u;
-    beg = xcd * q + (xcd < r ? xcd : r);
-    end = beg + q + (xcd < r ? 1u : 0u);
-}
-
 // Waves per SIMD the raster pass is compiled for: latency hiding pays for a few
 // spilled registers with the light shaders; the shadow shaders would spill too
 // much and keep the default.
@@ -753,8 +806,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     if (p.ctr->flags & FLAG_BIN_OVERFLOW) return;
     const uint32_t tid = threadIdx.x;
     const uint32_t ntiles = p.ntiles;
-    if (blockIdx.x == 0u && tid == 0u) p.ctr->ovf[(p.frame + 1u) & 1u][0] = 0u;  // next frame's list
-    const uint32_t novf = p.ctr->ovf[p.frame & 1u][0] < p.ovf_cap ? p.ctr->ovf[p.frame & 1u][0] : p.ovf_cap;
+    const uint32_t par = p.frame & 1u, npar = (p.frame + 1u) & 1u;
+    if (blockIdx.x == 0u && tid == 0u) p.ctr->ovf[npar][0] = 0u;  // next frame's list
+    const uint32_t novf = p.ctr->ovf[par][0] < p.ovf_cap ? p.ctr->ovf[par][0] : p.ovf_cap;
     const bool uses_depth = p.depth_test != DT_NONE || p.depth_write;
     const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
                         : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
@@ -764,23 +818,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     const uint32_t xcd = blockIdx.x & 7u;
     uint32_t t0, t1;
     xcd_range(ntiles, xcd, t0, t1);
-    const uint32_t per_xcd = gridDim.x >> 3;
-    uint32_t *next = &p.ctr->next[p.frame & 1u][xcd][0];
-    if (blockIdx.x < 8u && tid == 0u) p.ctr->next[(p.frame + 1u) & 1u][xcd][0] = 0u;  // next frame's queue
-    // Workgroup b first takes heavy tile b (if there is one), then its tiles as
-    // above; a tile is claimed by exchanging its count, and the heavy pass leaves
-    // kTileTaken so the regular pass skips it (and zeroes the count for the next frame).
-    // Batches skip this: their tiles are mostly busy and keep their order.
-    bool heavy_pass = false;
-    uint32_t tile = t0 + (blockIdx.x >> 3);
-    if constexpr (!BATCH) {
-        if (blockIdx.x == 0u && tid == 0u) p.ctr->heavy[(p.frame + 1u) & 1u][0] = 0u;
-        uint32_t nh = __builtin_amdgcn_readfirstlane(p.ctr->heavy[p.frame & 1u][0]);
-        nh = nh < p.heavy_cap ? nh : p.heavy_cap;
-        heavy_pass = blockIdx.x < nh;
-        if (heavy_pass) tile = __builtin_amdgcn_readfirstlane(p.heavy[blockIdx.x]);
-    }
-    while (heavy_pass || tile < t1) {
+    const uint32_t per_xcd = gridDim.x >> 3, w = blockIdx.x >> 3;
+    uint32_t *next = &p.ctr->next[par][xcd][0];
+    if (blockIdx.x < 8u && tid == 0u) p.ctr->next[npar][xcd][0] = 0u;  // next frame's queue
+
+    // One tile with `cnt` bin entries: winner keys in LDS, then the resolve.
+    // Ends with a barrier (LDS is reused by the next tile).
+    auto run_tile = [&](uint32_t tile, uint32_t cnt) {
         uint32_t r = 0;
         if constexpr (BATCH) r = __builtin_amdgcn_readfirstlane(p.tile_req[tile]);
         const View &v = view_of<BATCH>(p, r);
@@ -804,84 +848,130 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         }
         const uint32_t cap = v.tile_cap;
         const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
-        uint32_t cnt;
-        if constexpr (BATCH) {
-            cnt = p.tile_count[tile];
-            __syncthreads();
-            if (tid == 0u) {
-                p.tile_count[tile] = 0u;  // counter of the next frame
-                if (cnt > cap) atomicMax(p.req_max + r, cnt);
+        if (tid == 0u && cnt > cap) atomicMax(p.req_max + r, cnt);
+        __syncthreads();
+        // this tile's bin, then (only when it overflowed) its entries of the overflow list
+        const uint32_t nb = cnt < cap ? cnt : cap;
+        const uint32_t nscan = nb + (cnt > cap ? novf : 0u);
+        for (uint32_t b = 0; b < nscan; b += kBatch) {
+            const uint32_t i = b + tid;
+            bool valid = false;
+            uint32_t prim = 0;
+            if (tid >= kBatch || i >= nscan) {
+            } else if (i < nb) {
+                valid = true, prim = bin[i];
+            } else {
+                const uint2 e = p.ovf[i - nb];
+                valid = e.x == tile, prim = e.y;
             }
-        } else {
-            if (tid == 0u) {
-                uint32_t c = atomicExch(p.tile_count + tile, heavy_pass ? kTileTaken : 0u);
-                if (heavy_pass && c == 0u) {  // the regular pass took it first
-                    p.tile_count[tile] = 0u;
-                    c = kTileTaken;
-                }
-                sh.cnt = c;
-                if (c != kTileTaken && c > cap) atomicMax(p.req_max + r, c);
-            }
-            __syncthreads();
-            cnt = sh.cnt;
+            raster_batch(p, v, sh, tid, valid, prim, g, mode);
         }
-        if (cnt != kTileTaken) {
-            // this tile's bin, then (only when it overflowed) its entries of the overflow list
-            const uint32_t nb = cnt < cap ? cnt : cap;
-            const uint32_t nscan = nb + (cnt > cap ? novf : 0u);
-            for (uint32_t b = 0; b < nscan; b += kBatch) {
-                const uint32_t i = b + tid;
-                bool valid = false;
-                uint32_t prim = 0;
-                if (tid >= kBatch) {
-                } else if (i < nb) {
-                    valid = true, prim = bin[i];
-                } else if (i < nscan) {
-                    const uint2 e = p.ovf[i - nb];
-                    valid = e.x == tile, prim = e.y;
-                }
-                raster_batch(p, v, sh, tid, valid, prim, g, mode);
+        if (inside) {
+            const uint64_t kk = sh.key[tid];
+            bool has;
+            uint32_t prim;
+            float z;
+            if (mode == 0u) {
+                has = kk != 0ull;
+                prim = (uint32_t)(kk >> 32) - 1u;
+                z = __uint_as_float((uint32_t)kk);
+            } else {
+                has = ((uint32_t)kk) != 0u;
+                prim = ((uint32_t)kk >> 1) - 1u;
+                const uint32_t o = (uint32_t)(kk >> 32);
+                z = zunord(mode == 1u ? o : ~o, (uint32_t)kk & 1u);
             }
-            if (inside) {
-                const uint64_t kk = sh.key[tid];
-                bool has;
-                uint32_t prim;
-                float z;
-                if (mode == 0u) {
-                    has = kk != 0ull;
-                    prim = (uint32_t)(kk >> 32) - 1u;
-                    z = __uint_as_float((uint32_t)kk);
-                } else {
-                    has = ((uint32_t)kk) != 0u;
-                    prim = ((uint32_t)kk >> 1) - 1u;
-                    const uint32_t o = (uint32_t)(kk >> 32);
-                    z = zunord(mode == 1u ? o : ~o, (uint32_t)kk & 1u);
+            if (v.pixel) {
+                if (p.pixel_write && has && SH != DEPTH_ONLY) {
+                    TriRec R;
+                    load_rec0(p, v.tri_base + prim, R);
+                    tri_sids(v, prim, R.flags, R.sid);
+                    store_stream(&v.pixel[pidx], shade_pixel<SH>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
+                                                    v.clear_color_en ? v.clear_color : 0u));
+                } else if (v.clear_color_en) {
+                    store_stream(&v.pixel[pidx], v.clear_color);
                 }
-                if (v.pixel) {
-                    if (p.pixel_write && has && SH != DEPTH_ONLY) {
-                        TriRec R;
-                        load_rec0(p, v.tri_base + prim, R);
-                        tri_sids(v, prim, R.flags, R.sid);
-                        store_stream(&v.pixel[pidx], shade_pixel<SH>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
-                                                        v.clear_color_en ? v.clear_color : 0u));
-                    } else if (v.clear_color_en) {
-                        store_stream(&v.pixel[pidx], v.clear_color);
+            }
+            if (v.depth) {
+                if (has && p.depth_write) store_stream(&v.depth[pidx], z);
+                else if (v.clear_depth_en) store_stream(&v.depth[pidx], v.clear_depth);
+            }
+        }
+        __syncthreads();
+    };
+
+    if constexpr (!BATCH) {
+        if (p.busy) {
+            // The next frame's counts start at zero (stores only, issued first).
+            for (uint32_t i = blockIdx.x * 256u + tid; i < p.zero_n; i += gridDim.x * 256u) p.count_next[i] = 0u;
+            if (blockIdx.x == 0u && tid == 0u) p.ctr->heavy[npar][0] = 0u;
+            if (blockIdx.x < 8u && tid == 0u) p.ctr->busy[npar][xcd][0] = 0u;
+            // this workgroup's busy list: its XCD's (index w of per_xcd), or the one list
+            const uint32_t bl = kBusyMap == 1u ? 0u : xcd;
+            const uint32_t bw = kBusyMap == 1u ? blockIdx.x : w;
+            const uint32_t bper = kBusyMap == 1u ? gridDim.x : per_xcd;
+            uint32_t *bnext = &p.ctr->next[par][bl][0];
+            // list lengths and this workgroup's first busy tile, loaded together
+            // (the entry is only used when it is inside the list)
+            uint32_t nh = __builtin_amdgcn_readfirstlane(p.ctr->heavy[par][0]);
+            uint32_t nbusy = __builtin_amdgcn_readfirstlane(p.ctr->busy[par][bl][0]);
+            const uint32_t *list = p.busy + (size_t)bl * ntiles;
+            const uint32_t first = bw < ntiles ? __builtin_amdgcn_readfirstlane(list[bw]) : 0u;
+            nh = nh < p.heavy_cap ? nh : p.heavy_cap;
+            nbusy = nbusy < ntiles ? nbusy : ntiles;
+            // 1. Heavy tiles first, workgroup k taking heavy[k] (the longest
+            //    tiles start early instead of ending the launch late).
+            for (uint32_t k = blockIdx.x; k < nh; k += gridDim.x) {
+                const uint32_t tile = __builtin_amdgcn_readfirstlane(p.heavy[k]);
+                run_tile(tile, __builtin_amdgcn_readfirstlane(p.tile_count[tile]));
+            }
+            // 2. Busy tiles of this XCD from its queue (heavy ones are done).
+            for (uint32_t k = bw; k < nbusy;) {
+                const uint32_t tile = k == bw ? first : __builtin_amdgcn_readfirstlane(list[k]);
+                const uint32_t cnt = __builtin_amdgcn_readfirstlane(p.tile_count[tile]);
+                if (!(p.heavy_cap && cnt >= kHeavyTile)) run_tile(tile, cnt);
+                if (tid == 0u) sh.next = bper + atomicAdd(bnext, 1u);
+                __syncthreads();
+                k = __builtin_amdgcn_readfirstlane(sh.next);
+                __syncthreads();
+            }
+            // 3. Empty tiles of this workgroup's share (every per_xcd-th tile of
+            //    the XCD's range): the clears, stores only, no LDS.
+            const View &v = p.v0;
+            if (v.clear_color_en || v.clear_depth_en) {
+                for (uint32_t base = t0 + w; base < t1; base += 8u * per_xcd) {
+                    uint32_t c[8];
+#pragma unroll
+                    for (uint32_t j = 0; j < 8u; j++) {
+                        const uint32_t t = base + j * per_xcd;
+                        c[j] = t < t1 ? p.tile_count[t] : 1u;
+                    }
+#pragma unroll
+                    for (uint32_t j = 0; j < 8u; j++) {
+                        if (c[j] != 0u) continue;
+                        const uint32_t lt = base + j * per_xcd;
+                        const uint32_t ty = lt / v.ntx, tx = lt - ty * v.ntx;
+                        const uint32_t px = (tx << sx) + (tid & ((1u << sx) - 1u)), py = (ty << sy) + (tid >> sx);
+                        if (px < v.W && py < v.H) {
+                            const size_t pidx = (size_t)py * v.W + px;
+                            if (v.pixel && v.clear_color_en) store_stream(&v.pixel[pidx], v.clear_color);
+                            if (v.depth && v.clear_depth_en) store_stream(&v.depth[pidx], v.clear_depth);
+                        }
                     }
                 }
-                if (v.depth) {
-                    if (has && p.depth_write) store_stream(&v.depth[pidx], z);
-                    else if (v.clear_depth_en) store_stream(&v.depth[pidx], v.clear_depth);
-                }
             }
-        }  // cnt != kTileTaken
-        if (heavy_pass) {
-            heavy_pass = false;
-            tile = t0 + (blockIdx.x >> 3);
-            __syncthreads();  // LDS is reused by the next tile
-            continue;
+            return;
         }
-        if (tid == 0u) sh.next = t0 + per_xcd + atomicAdd(next, 1u);
-        __syncthreads();  // LDS is reused by the next tile
+    }
+    // Every tile from the queue; its count is re-zeroed for the next launch.
+    for (uint32_t tile = t0 + w; tile < t1;) {
+        const uint32_t cnt = __builtin_amdgcn_readfirstlane(p.tile_count[tile]);
+        run_tile(tile, cnt);
+        if (tid == 0u) {
+            p.tile_count[tile] = 0u;
+            sh.next = t0 + per_xcd + atomicAdd(next, 1u);
+        }
+        __syncthreads();
         tile = __builtin_amdgcn_readfirstlane(sh.next);
         __syncthreads();
     }

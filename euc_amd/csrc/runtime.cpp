@@ -55,6 +55,8 @@ struct euchip_ctx {
     DevArray<uint32_t> tile_count, bins;
     DevArray<uint2> ovf;
     DevArray<uint32_t> heavy;   // heavy-tile list
+    DevArray<uint32_t> busy;    // busy-tile lists (one per XCD tile range)
+    uint32_t counts_used[2] = {0, 0};  // tiles of the last frame that counted into each half of tile_count
     uint32_t tile_cap = 128;    // bin slots per tile; raised when a frame needed more
     uint32_t frame = 0;
     uint32_t max_groups = 2048; // resident raster workgroups: CUs x 8
@@ -395,6 +397,7 @@ euchip_status after_frame(euchip_ctx *ctx, bool &dropped) {
         HIP_TRY(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(Counters), ctx->stream));
         HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->counts_used[0] = ctx->counts_used[1] = 0;
     }
     return EUCHIP_OK;
 }
@@ -427,12 +430,18 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     if ((s = ensure(ctx, ctx->rec1, std::max<size_t>(P.n_tris * kRec1Quads, 1), "setup records", ctx->stream))) return s;
     if ((s = ensure(ctx, ctx->vary, std::max<size_t>(nsv * vd_count(d->shader), 1), "vertex data", ctx->stream)))
         return s;
-    const bool fresh_tiles = ctx->tile_count.cap < ntiles;
-    if ((s = ensure(ctx, ctx->tile_count, ntiles, "tile counts", ctx->stream))) return s;
-    if (fresh_tiles) HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4, ctx->stream));
+    // two halves: setup counts into one while the raster pass zeroes the other
+    const bool fresh_tiles = ctx->tile_count.cap < 2 * ntiles;
+    if ((s = ensure(ctx, ctx->tile_count, 2 * ntiles, "tile counts", ctx->stream))) return s;
+    if (fresh_tiles) {
+        HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4, ctx->stream));
+        ctx->counts_used[0] = ctx->counts_used[1] = 0;
+    }
+    const size_t half = ctx->tile_count.cap / 2;
     if ((s = ensure(ctx, ctx->ovf, std::max<size_t>((size_t)1 << 20, (size_t)P.n_tris), "overflow list", ctx->stream)))
         return s;
     if ((s = ensure(ctx, ctx->heavy, ntiles, "heavy tiles", ctx->stream))) return s;
+    if ((s = ensure(ctx, ctx->busy, 8 * ntiles, "busy tiles", ctx->stream))) return s;
 
     for (int attempt = 0;; attempt++) {
         if ((s = ensure(ctx, ctx->bins, ntiles * ctx->tile_cap, "bins", ctx->stream))) return s;
@@ -441,7 +450,11 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.vary = ctx->vary.ptr;
         p.ctr = ctx->counters;
         p.req_max = &ctx->counters->max_count;
-        p.tile_count = ctx->tile_count.ptr;
+        const uint32_t par = ctx->frame & 1u, npar = par ^ 1u;
+        p.tile_count = ctx->tile_count.ptr + par * half;
+        p.count_next = ctx->tile_count.ptr + npar * half;
+        p.zero_n = ctx->counts_used[npar];
+        p.busy = ctx->busy.ptr;
         p.bins = ctx->bins.ptr;
         p.v0.tile_cap = ctx->tile_cap;
         p.v0.bin_base = 0;
@@ -452,6 +465,8 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.frame = ctx->frame++;
         if ((s = timed(ctx, "setup", [&] { return launch_setup(p, ctx->stream, false, 0); }))) return s;
         if ((s = timed(ctx, "raster", [&] { return launch_raster(p, ctx->stream, false, ctx->max_groups); }))) return s;
+        ctx->counts_used[npar] = 0;
+        ctx->counts_used[par] = (uint32_t)ntiles;
         if (!sync) return EUCHIP_OK;
         HIP_TRY(ctx, hipMemcpyAsync(ctx->host_counters, ctx->counters, sizeof(Counters), hipMemcpyDeviceToHost,
                                     ctx->stream));
@@ -603,6 +618,10 @@ euchip_status build_tables(euchip_batch *b) {
     p.ovf_cap = (uint32_t)std::min<size_t>(b->ovf.cap, 0xFFFFFFF0ull);
     p.heavy = nullptr;  // batches keep their tile order: most of their tiles would be heavy
     p.heavy_cap = 0;
+    p.busy = nullptr;
+    p.count_next = nullptr;
+    p.zero_n = 0;
+
     p.frame = 0;
     p.ntiles = (uint32_t)tiles;
     p.tile_sx = sx;
@@ -797,6 +816,8 @@ euchip_status euchip_ctx_destroy(euchip_ctx *ctx) {
     release(ctx, ctx->bins);
     release(ctx, ctx->ovf);
     release(ctx, ctx->heavy);
+    release(ctx, ctx->busy);
+
     (void)hipFree(ctx->counters);
     if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
