@@ -803,22 +803,39 @@ constexpr uint32_t raster_waves(uint32_t sh) {
 template <uint32_t SH, bool BATCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(SH)))) void k_raster(FrameParams p) {
     __shared__ TileShared sh;
-    if (p.ctr->flags & FLAG_BIN_OVERFLOW) return;
     const uint32_t tid = threadIdx.x;
     const uint32_t ntiles = p.ntiles;
     const uint32_t par = p.frame & 1u, npar = (p.frame + 1u) & 1u;
+    const uint32_t xcd = blockIdx.x & 7u;
+    const uint32_t per_xcd = gridDim.x >> 3, w = blockIdx.x >> 3;
+    // The loads the first tile waits for, issued together: frame flags, overflow
+    // count and (one-request frames) list lengths, this workgroup's first busy
+    // tile and its count (read at a clamped index before the tile is known to
+    // be in the list; used only if it is).
+    const uint32_t bl = kBusyMap == 1u ? 0u : xcd;
+    const uint32_t bw = kBusyMap == 1u ? blockIdx.x : w;
+    const uint32_t bper = kBusyMap == 1u ? gridDim.x : per_xcd;
+    const uint32_t *list = BATCH ? nullptr : p.busy + (size_t)bl * ntiles;
+    const uint32_t flags = p.ctr->flags;
+    const uint32_t novf0 = p.ctr->ovf[par][0];
+    uint32_t nh = 0u, nbusy = 0u, first = 0u, first_cnt = 0u;
+    if (!BATCH && p.busy) {
+        nh = __builtin_amdgcn_readfirstlane(p.ctr->heavy[par][0]);
+        nbusy = __builtin_amdgcn_readfirstlane(p.ctr->busy[par][bl][0]);
+        first = bw < ntiles ? __builtin_amdgcn_readfirstlane(list[bw]) : 0u;
+        first_cnt = __builtin_amdgcn_readfirstlane(p.tile_count[first < ntiles ? first : 0u]);
+    }
+    if (flags & FLAG_BIN_OVERFLOW) return;
     if (blockIdx.x == 0u && tid == 0u) p.ctr->ovf[npar][0] = 0u;  // next frame's list
-    const uint32_t novf = p.ctr->ovf[par][0] < p.ovf_cap ? p.ctr->ovf[par][0] : p.ovf_cap;
+    const uint32_t novf = novf0 < p.ovf_cap ? novf0 : p.ovf_cap;
     const bool uses_depth = p.depth_test != DT_NONE || p.depth_write;
     const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
                         : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
     const uint32_t sx = p.tile_sx, sy = 8u - sx;
     // Tiles of this XCD's range: the first by workgroup index, the rest from a
     // per-XCD counter, so workgroups that drew heavy tiles take fewer of them.
-    const uint32_t xcd = blockIdx.x & 7u;
     uint32_t t0, t1;
     xcd_range(ntiles, xcd, t0, t1);
-    const uint32_t per_xcd = gridDim.x >> 3, w = blockIdx.x >> 3;
     uint32_t *next = &p.ctr->next[par][xcd][0];
     if (blockIdx.x < 8u && tid == 0u) p.ctr->next[npar][xcd][0] = 0u;  // next frame's queue
 
@@ -907,16 +924,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             if (blockIdx.x == 0u && tid == 0u) p.ctr->heavy[npar][0] = 0u;
             if (blockIdx.x < 8u && tid == 0u) p.ctr->busy[npar][xcd][0] = 0u;
             // this workgroup's busy list: its XCD's (index w of per_xcd), or the one list
-            const uint32_t bl = kBusyMap == 1u ? 0u : xcd;
-            const uint32_t bw = kBusyMap == 1u ? blockIdx.x : w;
-            const uint32_t bper = kBusyMap == 1u ? gridDim.x : per_xcd;
             uint32_t *bnext = &p.ctr->next[par][bl][0];
-            // list lengths and this workgroup's first busy tile, loaded together
-            // (the entry is only used when it is inside the list)
-            uint32_t nh = __builtin_amdgcn_readfirstlane(p.ctr->heavy[par][0]);
-            uint32_t nbusy = __builtin_amdgcn_readfirstlane(p.ctr->busy[par][bl][0]);
-            const uint32_t *list = p.busy + (size_t)bl * ntiles;
-            const uint32_t first = bw < ntiles ? __builtin_amdgcn_readfirstlane(list[bw]) : 0u;
             nh = nh < p.heavy_cap ? nh : p.heavy_cap;
             nbusy = nbusy < ntiles ? nbusy : ntiles;
             // 1. Heavy tiles first, workgroup k taking heavy[k] (the longest
@@ -928,7 +936,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             // 2. Busy tiles of this XCD from its queue (heavy ones are done).
             for (uint32_t k = bw; k < nbusy;) {
                 const uint32_t tile = k == bw ? first : __builtin_amdgcn_readfirstlane(list[k]);
-                const uint32_t cnt = __builtin_amdgcn_readfirstlane(p.tile_count[tile]);
+                const uint32_t cnt = k == bw ? first_cnt : __builtin_amdgcn_readfirstlane(p.tile_count[tile]);
                 if (!(p.heavy_cap && cnt >= kHeavyTile)) run_tile(tile, cnt);
                 if (tid == 0u) sh.next = bper + atomicAdd(bnext, 1u);
                 __syncthreads();
@@ -1013,28 +1021,52 @@ hipError_t launch_setup(const FrameParams &p, hipStream_t s, bool batched, uint3
     return hipGetLastError();
 }
 
+// Workgroups of the one-request raster kernel that fit on the device at once
+// (`cus` compute units); its busy-tile loop gives every workgroup a first tile,
+// so workgroups that could only start when others finish would hold theirs back.
 template <uint32_t SH>
-static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, uint32_t groups) {
-    if (batched) hipLaunchKernelGGL((k_raster<SH, true>), dim3(groups), dim3(256), 0, s, p);
-    else hipLaunchKernelGGL((k_raster<SH, false>), dim3(groups), dim3(256), 0, s, p);
+static uint32_t resident_groups(uint32_t cus) {
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_raster<SH, false>), 256, 0) !=
+                hipSuccess || n < 1)
+            n = 8;
+        per_cu = n;
+    }
+    return (uint32_t)per_cu * cus;
+}
+
+template <uint32_t SH>
+static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, uint32_t groups, uint32_t cus) {
+    if (batched) {
+        hipLaunchKernelGGL((k_raster<SH, true>), dim3(groups), dim3(256), 0, s, p);
+    } else {
+        if (p.busy) {
+            const uint32_t res = std::max<uint32_t>(8u, resident_groups<SH>(cus) / 8u * 8u);
+            groups = std::min(groups, res);
+        }
+        hipLaunchKernelGGL((k_raster<SH, false>), dim3(groups), dim3(256), 0, s, p);
+    }
 }
 
 // Workgroups of the tile loop: a multiple of the 8 XCDs, no more than the
-// tiles need (rounded up to 8) and no more than `max_groups`.
+// tiles need (rounded up to 8) and no more than `max_groups` (compute units x 8).
 hipError_t launch_raster(const FrameParams &p, hipStream_t s, bool batched, uint32_t max_groups) {
     if (p.ntiles == 0) return hipSuccess;
     uint32_t g = ((p.ntiles + 7u) / 8u) * 8u;
     const uint32_t cap = std::max<uint32_t>(8u, (max_groups / 8u) * 8u);
     if (g > cap) g = cap;
+    const uint32_t cus = std::max<uint32_t>(1u, max_groups / 8u);
     switch (p.shader) {
-    case FLAT: launch_raster_t<FLAT>(p, s, batched, g); break;
-    case COLOR: launch_raster_t<COLOR>(p, s, batched, g); break;
-    case GOURAUD: launch_raster_t<GOURAUD>(p, s, batched, g); break;
-    case PHONG: launch_raster_t<PHONG>(p, s, batched, g); break;
-    case DEPTH_ONLY: launch_raster_t<DEPTH_ONLY>(p, s, batched, g); break;
-    case TEXTURED: launch_raster_t<TEXTURED>(p, s, batched, g); break;
-    case PHONG_SHADOW: launch_raster_t<PHONG_SHADOW>(p, s, batched, g); break;
-    case TEX_SHADOW: launch_raster_t<TEX_SHADOW>(p, s, batched, g); break;
+    case FLAT: launch_raster_t<FLAT>(p, s, batched, g, cus); break;
+    case COLOR: launch_raster_t<COLOR>(p, s, batched, g, cus); break;
+    case GOURAUD: launch_raster_t<GOURAUD>(p, s, batched, g, cus); break;
+    case PHONG: launch_raster_t<PHONG>(p, s, batched, g, cus); break;
+    case DEPTH_ONLY: launch_raster_t<DEPTH_ONLY>(p, s, batched, g, cus); break;
+    case TEXTURED: launch_raster_t<TEXTURED>(p, s, batched, g, cus); break;
+    case PHONG_SHADOW: launch_raster_t<PHONG_SHADOW>(p, s, batched, g, cus); break;
+    case TEX_SHADOW: launch_raster_t<TEX_SHADOW>(p, s, batched, g, cus); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
