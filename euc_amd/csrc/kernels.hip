@@ -400,17 +400,6 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
     }
     const View &v = view_of<BATCH>(p, r);
     const uint32_t t = j * 256u + tid;
-    if constexpr (vd_count(SH) > 0) {
-        if (t < v.n_instances * v.n_vertices) {
-            const uint32_t inst = t / v.n_vertices;
-            const uint32_t vi = t - inst * v.n_vertices;
-            const size_t sv = (size_t)v.sv_base + t;
-            float vd[VDN<SH>::N];
-            vertex_data<SH>(p, v, inst, vi, vd);
-#pragma unroll
-            for (uint32_t q = 0; q < vd_count(SH); q++) p.vary[sv * vd_count(SH) + q] = vd[q];
-        }
-    }
     const uint32_t sx = p.tile_sx, sy = 8u - p.tile_sx;
     bool keep = false;
     uint32_t tx0 = 0, ty0 = 0, nx = 0, ny = 0;
@@ -433,6 +422,19 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
         }
     }
     sm.rec.kept[tid] = keep ? 1u : 0u;
+    // The vertex stage after the triangles' loads: the VertexData stores would
+    // otherwise order those loads behind them.
+    if constexpr (vd_count(SH) > 0) {
+        if (t < v.n_instances * v.n_vertices) {
+            const uint32_t inst = t / v.n_vertices;
+            const uint32_t vi = t - inst * v.n_vertices;
+            const size_t sv = (size_t)v.sv_base + t;
+            float vd[VDN<SH>::N];
+            vertex_data<SH>(p, v, inst, vi, vd);
+#pragma unroll
+            for (uint32_t q = 0; q < vd_count(SH); q++) p.vary[sv * vd_count(SH) + q] = vd[q];
+        }
+    }
     __syncthreads();
     // the workgroup's records are contiguous: write them out 16 bytes per lane,
     // skipping the culled triangles' slots
