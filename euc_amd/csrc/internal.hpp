@@ -67,10 +67,11 @@ constexpr uint32_t kBatch = 128;
 // Tiles whose bin reaches this many entries are listed by setup and rasterised
 // first (the longest tiles start early instead of ending the launch late).
 constexpr uint32_t kHeavyTile = 160u;
-// Busy-tile lists of one-request frames: 0 = one per XCD's contiguous tile
-// range, 1 = one list, 2 = blocks of 4 x 4 tiles dealt over the 8 XCDs (each
-// XCD gets a share of every screen region, so dense regions do not load one XCD).
-constexpr uint32_t kBusyMap = 2;
+// Which of the 8 busy-tile lists (one per XCD) a tile goes to: its XCD's
+// contiguous tile range (batches: each XCD gets whole requests), or its block
+// of 4 x 4 tiles dealt over the XCDs (one-request frames: each XCD gets a share
+// of every screen region, so dense regions do not load one XCD).
+enum BusyMap : uint32_t { BUSY_RANGES = 0, BUSY_BLOCKS = 1 };
 
 // One render request: a target pair, a mesh, a uniform block, and where the
 // request's vertices, tiles and bins sit in the launch's scratch memory.
@@ -116,21 +117,21 @@ struct FrameParams {
     uint4 *rec1;           // kRec1Quads per triangle of the launch
     float *vary;           // VertexData per shaded vertex
     Counters *ctr;
-    uint32_t *tile_count;  // per tile of the launch, zero before setup
+    uint32_t *tile_count;  // per tile of the launch, zero before setup, read-only in raster
     uint32_t *bins;        // bin_base + local tile * tile_cap + rank -> primitive
     uint32_t *req_max;     // per request: largest tile count above its tile_cap (host grows the bins)
     uint2 *ovf;            // (tile, primitive) of ranks >= tile_cap
     uint32_t ovf_cap;
     uint32_t *heavy;       // tiles with at least kHeavyTile entries (heavy_cap slots)
     uint32_t heavy_cap;    // 0: no heavy-first pass
-    // One-request frames: setup lists the tiles it bins into (busy) in 8 lists
-    // of up to ntiles entries (list x at busy + x * ntiles, see kBusyMap); raster
-    // clears the empty tiles itself, takes only busy tiles from its queues,
-    // leaves `tile_count` as it is and zeroes the first `zero_n` counts of
-    // `count_next` (the other half of a two-frame count buffer).  busy ==
-    // nullptr: one tile queue per XCD range, and raster re-zeroes each count it
-    // reads (batches).
+    // Setup lists the tiles it bins into (busy) in 8 lists of up to ntiles
+    // entries (list x at busy + x * ntiles, chosen by busy_map); raster clears
+    // the empty tiles itself, takes only busy tiles from the lists, leaves
+    // `tile_count` as it is and zeroes the first `zero_n` counts of
+    // `count_next` (one-request frames: the other half of a two-frame count
+    // buffer; batches zero their counts with a fill before setup).
     uint32_t *busy;
+    uint32_t busy_map;
     uint32_t *count_next;
     uint32_t zero_n;
     uint32_t frame;        // frame parity selects the overflow counter and tile queue

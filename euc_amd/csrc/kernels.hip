@@ -224,20 +224,28 @@ EUCHIP_DEV void xcd_range(uint32_t ntiles, uint32_t xcd, uint32_t &beg, uint32_t
     end = beg + q + (xcd < r ? 1u : 0u);
 }
 
-// Busy list of a tile (list x holds up to ntiles entries at p.busy + x * ntiles).
-// kBusyMap 0: the XCD whose contiguous tile range holds the tile; 1: one list
-// for the whole launch; 2: blocks of 4 x 4 tiles dealt over the 8 XCDs.
+// Busy list x: BUSY_BLOCKS lists hold up to ntiles entries at p.busy + x *
+// ntiles; BUSY_RANGES list x holds the tiles of range x in that range's own slots.
+EUCHIP_DEV void busy_list_span(const FrameParams &p, uint32_t x, uint32_t &base, uint32_t &cap) {
+    if (p.busy_map == BUSY_BLOCKS) {
+        base = x * p.ntiles, cap = p.ntiles;
+    } else {
+        uint32_t e;
+        xcd_range(p.ntiles, x, base, e);
+        cap = e - base;
+    }
+}
+
+// Busy list of a tile.
 EUCHIP_DEV uint32_t busy_list_of(const FrameParams &p, uint32_t tile) {
-    if constexpr (kBusyMap == 1u) {
-        return 0u;
-    } else if constexpr (kBusyMap == 2u) {
+    if (p.busy_map == BUSY_BLOCKS) {  // one request: blocks of 4 x 4 tiles dealt over the XCDs
         const uint32_t ty = tile / p.v0.ntx, tx = tile - ty * p.v0.ntx;
         return ((tx >> 2) + 3u * (ty >> 2)) & 7u;
-    } else {
-        const uint32_t q = p.ntiles >> 3, r = p.ntiles & 7u;
-        const uint32_t big = r * (q + 1u);  // tiles of the ranges one longer
-        return tile < big ? tile / (q + 1u) : r + (tile - big) / q;
     }
+    // the XCD whose contiguous tile range (xcd_range) holds the tile
+    const uint32_t q = p.ntiles >> 3, r = p.ntiles & 7u;
+    const uint32_t big = r * (q + 1u);  // tiles of the ranges one longer
+    return tile < big ? tile / (q + 1u) : r + (tile - big) / q;
 }
 
 // A tile whose count went from `old` to `old + n` may have become heavy.
@@ -254,8 +262,10 @@ EUCHIP_DEV void bin_direct(const FrameParams &p, const View &v, uint32_t tile, u
     note_heavy(p, tile, rank, 1u);
     if (p.busy && rank == 0u) {
         const uint32_t x = busy_list_of(p, tile);
+        uint32_t lb, lc;
+        busy_list_span(p, x, lb, lc);
         const uint32_t k = atomicAdd(&p.ctr->busy[p.frame & 1u][x][0], 1u);
-        if (k < p.ntiles) p.busy[(size_t)x * p.ntiles + k] = tile;
+        if (k < lc) p.busy[(size_t)lb + k] = tile;
     }
     bin_store(p, v, tile, rank, prim);
 }
@@ -497,10 +507,22 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
         }
     }
     __syncthreads();
+    // list bases (one global atomic per list) in flight while the bins are written
+    uint32_t any_new = 0u;
+#pragma unroll
+    for (uint32_t q = 0; q < 9u; q++) any_new |= lcnt[q];
+    uint32_t my_base = 0u;
     if (tid < 9u && lcnt[tid]) {
-        lbase[tid] = tid < 8u ? atomicAdd(&p.ctr->busy[p.frame & 1u][tid][0], lcnt[tid])
-                              : atomicAdd(&p.ctr->heavy[p.frame & 1u][0], lcnt[tid]);
+        my_base = tid < 8u ? atomicAdd(&p.ctr->busy[p.frame & 1u][tid][0], lcnt[tid])
+                           : atomicAdd(&p.ctr->heavy[p.frame & 1u][0], lcnt[tid]);
     }
+    const uint32_t np = npl < kPairs ? npl : kPairs;
+    for (uint32_t k = tid; k < np; k += 256u) {
+        const uint32_t h = sm.bin.pl_slot[k];
+        bin_store(p, v, sm.bin.hkey[h] - 1u, sm.bin.hbase[h] + sm.bin.pl_local[k], sm.bin.pl_prim[k]);
+    }
+    if (!any_new) return;  // no tile became busy or heavy here (uniform)
+    if (tid < 9u) lbase[tid] = my_base;
     __syncthreads();
     for (uint32_t i = tid; i < kHash; i += 256u) {
         const uint32_t code = sm.bin.hcnt[i];
@@ -508,18 +530,14 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
         const uint32_t tile = sm.bin.hkey[i] - 1u;
         if (code & (1u << 14)) {
             const uint32_t x = (code >> 11) & 7u, k = lbase[x] + (code & 0x7ffu);
-            if (k < p.ntiles) p.busy[(size_t)x * p.ntiles + k] = tile;
+            uint32_t lb, lc;
+            busy_list_span(p, x, lb, lc);
+            if (k < lc) p.busy[(size_t)lb + k] = tile;
         }
         if (code & (1u << 26)) {
             const uint32_t k = lbase[8] + ((code >> 15) & 0x7ffu);
             if (k < p.heavy_cap) p.heavy[k] = tile;
         }
-    }
-    __syncthreads();
-    const uint32_t np = npl < kPairs ? npl : kPairs;
-    for (uint32_t k = tid; k < np; k += 256u) {
-        const uint32_t h = sm.bin.pl_slot[k];
-        bin_store(p, v, sm.bin.hkey[h] - 1u, sm.bin.hbase[h] + sm.bin.pl_local[k], sm.bin.pl_prim[k]);
     }
 }
 
@@ -814,19 +832,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     // count and (one-request frames) list lengths, this workgroup's first busy
     // tile and its count (read at a clamped index before the tile is known to
     // be in the list; used only if it is).
-    const uint32_t bl = kBusyMap == 1u ? 0u : xcd;
-    const uint32_t bw = kBusyMap == 1u ? blockIdx.x : w;
-    const uint32_t bper = kBusyMap == 1u ? gridDim.x : per_xcd;
-    const uint32_t *list = BATCH ? nullptr : p.busy + (size_t)bl * ntiles;
+    uint32_t lbase, lcap;
+    busy_list_span(p, xcd, lbase, lcap);
+    const uint32_t *list = p.busy + lbase;  // this XCD's busy list
     const uint32_t flags = p.ctr->flags;
     const uint32_t novf0 = p.ctr->ovf[par][0];
-    uint32_t nh = 0u, nbusy = 0u, first = 0u, first_cnt = 0u;
-    if (!BATCH && p.busy) {
-        nh = __builtin_amdgcn_readfirstlane(p.ctr->heavy[par][0]);
-        nbusy = __builtin_amdgcn_readfirstlane(p.ctr->busy[par][bl][0]);
-        first = bw < ntiles ? __builtin_amdgcn_readfirstlane(list[bw]) : 0u;
-        first_cnt = __builtin_amdgcn_readfirstlane(p.tile_count[first < ntiles ? first : 0u]);
-    }
+    uint32_t nh = __builtin_amdgcn_readfirstlane(p.ctr->heavy[par][0]);
+    uint32_t nbusy = __builtin_amdgcn_readfirstlane(p.ctr->busy[par][xcd][0]);
+    const uint32_t first = w < lcap ? __builtin_amdgcn_readfirstlane(list[w]) : 0u;
+    const uint32_t first_cnt = __builtin_amdgcn_readfirstlane(p.tile_count[first < ntiles ? first : 0u]);
     if (flags & FLAG_BIN_OVERFLOW) return;
     if (blockIdx.x == 0u && tid == 0u) p.ctr->ovf[npar][0] = 0u;  // next frame's list
     const uint32_t novf = novf0 < p.ovf_cap ? novf0 : p.ovf_cap;
@@ -834,10 +848,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
                         : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
     const uint32_t sx = p.tile_sx, sy = 8u - sx;
-    // Tiles of this XCD's range: the first by workgroup index, the rest from a
-    // per-XCD counter, so workgroups that drew heavy tiles take fewer of them.
-    uint32_t t0, t1;
-    xcd_range(ntiles, xcd, t0, t1);
     uint32_t *next = &p.ctr->next[par][xcd][0];
     if (blockIdx.x < 8u && tid == 0u) p.ctr->next[npar][xcd][0] = 0u;  // next frame's queue
 
@@ -919,71 +929,57 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         __syncthreads();
     };
 
-    if constexpr (!BATCH) {
-        if (p.busy) {
-            // The next frame's counts start at zero (stores only, issued first).
-            for (uint32_t i = blockIdx.x * 256u + tid; i < p.zero_n; i += gridDim.x * 256u) p.count_next[i] = 0u;
-            if (blockIdx.x == 0u && tid == 0u) p.ctr->heavy[npar][0] = 0u;
-            if (blockIdx.x < 8u && tid == 0u) p.ctr->busy[npar][xcd][0] = 0u;
-            // this workgroup's busy list: its XCD's (index w of per_xcd), or the one list
-            uint32_t *bnext = &p.ctr->next[par][bl][0];
-            nh = nh < p.heavy_cap ? nh : p.heavy_cap;
-            nbusy = nbusy < ntiles ? nbusy : ntiles;
-            // 1. Heavy tiles first, workgroup k taking heavy[k] (the longest
-            //    tiles start early instead of ending the launch late).
-            for (uint32_t k = blockIdx.x; k < nh; k += gridDim.x) {
-                const uint32_t tile = __builtin_amdgcn_readfirstlane(p.heavy[k]);
-                run_tile(tile, __builtin_amdgcn_readfirstlane(p.tile_count[tile]));
-            }
-            // 2. Busy tiles of this XCD from its queue (heavy ones are done).
-            for (uint32_t k = bw; k < nbusy;) {
-                const uint32_t tile = k == bw ? first : __builtin_amdgcn_readfirstlane(list[k]);
-                const uint32_t cnt = k == bw ? first_cnt : __builtin_amdgcn_readfirstlane(p.tile_count[tile]);
-                if (!(p.heavy_cap && cnt >= kHeavyTile)) run_tile(tile, cnt);
-                if (tid == 0u) sh.next = bper + atomicAdd(bnext, 1u);
-                __syncthreads();
-                k = __builtin_amdgcn_readfirstlane(sh.next);
-                __syncthreads();
-            }
-            // 3. Empty tiles of this workgroup's share (every per_xcd-th tile of
-            //    the XCD's range): the clears, stores only, no LDS.
-            const View &v = p.v0;
-            if (v.clear_color_en || v.clear_depth_en) {
-                for (uint32_t base = t0 + w; base < t1; base += 8u * per_xcd) {
-                    uint32_t c[8];
-#pragma unroll
-                    for (uint32_t j = 0; j < 8u; j++) {
-                        const uint32_t t = base + j * per_xcd;
-                        c[j] = t < t1 ? p.tile_count[t] : 1u;
-                    }
-#pragma unroll
-                    for (uint32_t j = 0; j < 8u; j++) {
-                        if (c[j] != 0u) continue;
-                        const uint32_t lt = base + j * per_xcd;
-                        const uint32_t ty = lt / v.ntx, tx = lt - ty * v.ntx;
-                        const uint32_t px = (tx << sx) + (tid & ((1u << sx) - 1u)), py = (ty << sy) + (tid >> sx);
-                        if (px < v.W && py < v.H) {
-                            const size_t pidx = (size_t)py * v.W + px;
-                            if (v.pixel && v.clear_color_en) store_stream(&v.pixel[pidx], v.clear_color);
-                            if (v.depth && v.clear_depth_en) store_stream(&v.depth[pidx], v.clear_depth);
-                        }
-                    }
-                }
-            }
-            return;
-        }
+    // The next frame's counts start at zero (stores only, issued first).
+    for (uint32_t i = blockIdx.x * 256u + tid; i < p.zero_n; i += gridDim.x * 256u) p.count_next[i] = 0u;
+    if (blockIdx.x == 0u && tid == 0u) p.ctr->heavy[npar][0] = 0u;
+    if (blockIdx.x < 8u && tid == 0u) p.ctr->busy[npar][xcd][0] = 0u;
+    nh = nh < p.heavy_cap ? nh : p.heavy_cap;
+    nbusy = nbusy < lcap ? nbusy : lcap;
+    // 1. Heavy tiles first, workgroup k taking heavy[k] (the longest tiles
+    //    start early instead of ending the launch late).
+    for (uint32_t k = blockIdx.x; k < nh; k += gridDim.x) {
+        const uint32_t tile = __builtin_amdgcn_readfirstlane(p.heavy[k]);
+        run_tile(tile, __builtin_amdgcn_readfirstlane(p.tile_count[tile]));
     }
-    // Every tile from the queue; its count is re-zeroed for the next launch.
-    for (uint32_t tile = t0 + w; tile < t1;) {
-        const uint32_t cnt = __builtin_amdgcn_readfirstlane(p.tile_count[tile]);
-        run_tile(tile, cnt);
-        if (tid == 0u) {
-            p.tile_count[tile] = 0u;
-            sh.next = t0 + per_xcd + atomicAdd(next, 1u);
+    // 2. Busy tiles of this XCD's list: the first by workgroup index, the rest
+    //    from the XCD's counter (heavy ones are done).
+    for (uint32_t k = w; k < nbusy;) {
+        const uint32_t tile = k == w ? first : __builtin_amdgcn_readfirstlane(list[k]);
+        const uint32_t cnt = k == w ? first_cnt : __builtin_amdgcn_readfirstlane(p.tile_count[tile]);
+        if (!(p.heavy_cap && cnt >= kHeavyTile)) run_tile(tile, cnt);
+        if (tid == 0u) sh.next = per_xcd + atomicAdd(next, 1u);
+        __syncthreads();
+        k = __builtin_amdgcn_readfirstlane(sh.next);
+        __syncthreads();
+    }
+    // 3. Empty tiles of this workgroup's share (every per_xcd-th tile of the
+    //    XCD's contiguous range): the clears, stores only, no LDS.
+    if (!BATCH && !p.v0.clear_color_en && !p.v0.clear_depth_en) return;
+    uint32_t t0, t1;
+    xcd_range(ntiles, xcd, t0, t1);
+    for (uint32_t base = t0 + w; base < t1; base += 8u * per_xcd) {
+        uint32_t c[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8u; j++) {
+            const uint32_t t = base + j * per_xcd;
+            c[j] = t < t1 ? p.tile_count[t] : 1u;
         }
-        __syncthreads();
-        tile = __builtin_amdgcn_readfirstlane(sh.next);
-        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < 8u; j++) {
+            if (c[j] != 0u) continue;
+            const uint32_t tile = base + j * per_xcd;
+            uint32_t r = 0;
+            if constexpr (BATCH) r = __builtin_amdgcn_readfirstlane(p.tile_req[tile]);
+            const View &v = view_of<BATCH>(p, r);
+            const uint32_t lt = tile - v.tile_base;
+            const uint32_t ty = lt / v.ntx, tx = lt - ty * v.ntx;
+            const uint32_t px = (tx << sx) + (tid & ((1u << sx) - 1u)), py = (ty << sy) + (tid >> sx);
+            if (px < v.W && py < v.H) {
+                const size_t pidx = (size_t)py * v.W + px;
+                if (v.pixel && v.clear_color_en) store_stream(&v.pixel[pidx], v.clear_color);
+                if (v.depth && v.clear_depth_en) store_stream(&v.depth[pidx], v.clear_depth);
+            }
+        }
     }
 }
 
@@ -1023,15 +1019,15 @@ hipError_t launch_setup(const FrameParams &p, hipStream_t s, bool batched, uint3
     return hipGetLastError();
 }
 
-// Workgroups of the one-request raster kernel that fit on the device at once
-// (`cus` compute units); its busy-tile loop gives every workgroup a first tile,
-// so workgroups that could only start when others finish would hold theirs back.
-template <uint32_t SH>
+// Workgroups of a raster kernel that fit on the device at once (`cus` compute
+// units); the busy-tile loop gives every workgroup a first tile, so workgroups
+// that could only start when others finish would hold theirs back.
+template <uint32_t SH, bool BATCH>
 static uint32_t resident_groups(uint32_t cus) {
     static int per_cu = 0;
     if (per_cu == 0) {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_raster<SH, false>), 256, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_raster<SH, BATCH>), 256, 0) !=
                 hipSuccess || n < 1)
             n = 8;
         per_cu = n;
@@ -1042,12 +1038,10 @@ static uint32_t resident_groups(uint32_t cus) {
 template <uint32_t SH>
 static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, uint32_t groups, uint32_t cus) {
     if (batched) {
+        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, true>(cus) / 8u * 8u));
         hipLaunchKernelGGL((k_raster<SH, true>), dim3(groups), dim3(256), 0, s, p);
     } else {
-        if (p.busy) {
-            const uint32_t res = std::max<uint32_t>(8u, resident_groups<SH>(cus) / 8u * 8u);
-            groups = std::min(groups, res);
-        }
+        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, false>(cus) / 8u * 8u));
         hipLaunchKernelGGL((k_raster<SH, false>), dim3(groups), dim3(256), 0, s, p);
     }
 }

@@ -110,10 +110,11 @@ struct euchip_batch {
     DevArray<uint4> tables;             // views, workgroup map and tile map, one block
     uint8_t *stage = nullptr;           // pinned staging of the tables
     size_t stage_bytes = 0;
-    bool counts_dirty = true;           // tile counters need a reset before the next launch
+    size_t n_tiles = 0;                 // tiles of the launch (their counts are zeroed before setup)
     DevArray<uint4> rec0, rec1;
     DevArray<float> vary;
     DevArray<uint32_t> tile_count, bins, ctrbuf;  // ctrbuf: Counters, then req_max per active request
+    DevArray<uint32_t> busy;                      // busy-tile lists (one slot per tile)
     DevArray<uint2> ovf;
     uint32_t *host_ctr = nullptr;       // pinned copy of ctrbuf
     size_t host_ctr_words = 0;
@@ -455,6 +456,7 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.count_next = ctx->tile_count.ptr + npar * half;
         p.zero_n = ctx->counts_used[npar];
         p.busy = ctx->busy.ptr;
+        p.busy_map = BUSY_BLOCKS;
         p.bins = ctx->bins.ptr;
         p.v0.tile_cap = ctx->tile_cap;
         p.v0.bin_base = 0;
@@ -575,9 +577,9 @@ euchip_status build_tables(euchip_batch *b) {
     if ((s = ensure(ctx, b->rec0, std::max<size_t>(tris * kRec0Quads, 1), "batch setup records", b->stream))) return s;
     if ((s = ensure(ctx, b->rec1, std::max<size_t>(tris * kRec1Quads, 1), "batch setup records", b->stream))) return s;
     if ((s = ensure(ctx, b->vary, std::max<size_t>(sv * vdc, 1), "batch vertex data", b->stream))) return s;
-    const uint32_t *old_counts = b->tile_count.ptr;
     if ((s = ensure(ctx, b->tile_count, std::max<size_t>(tiles, 1), "batch tile counts", b->stream))) return s;
-    if (b->tile_count.ptr != old_counts) b->counts_dirty = true;
+    if ((s = ensure(ctx, b->busy, std::max<size_t>(tiles, 1), "batch busy tiles", b->stream))) return s;
+    b->n_tiles = tiles;
     if ((s = ensure(ctx, b->bins, std::max<size_t>(bins, 1), "batch bins", b->stream))) return s;
     if ((s = ensure(ctx, b->ovf, std::max<size_t>((size_t)1 << 20, tris), "batch overflow list", b->stream))) return s;
     const size_t words = sizeof(Counters) / 4 + na;
@@ -593,12 +595,6 @@ euchip_status build_tables(euchip_batch *b) {
     // One copy from pinned memory; the staging area is rewritten only when the
     // batch's stream is idle again (set, set_tile_width and wait synchronise it).
     HIP_TRY(ctx, hipMemcpyAsync(b->tables.ptr, b->stage, table_bytes, hipMemcpyHostToDevice, b->stream));
-    // Tile counters are zero after every completed raster pass; they need a
-    // reset only when new or after a pass that stopped early.
-    if (b->counts_dirty) {
-        HIP_TRY(ctx, launch_fill_u32(b->tile_count.ptr, 0u, b->tile_count.cap, b->stream));
-        b->counts_dirty = false;
-    }
     const uint8_t *dev = reinterpret_cast<const uint8_t *>(b->tables.ptr);
     const View *d_views = reinterpret_cast<const View *>(dev);
     const uint2 *d_blk = reinterpret_cast<const uint2 *>(dev + off_blk);
@@ -618,7 +614,8 @@ euchip_status build_tables(euchip_batch *b) {
     p.ovf_cap = (uint32_t)std::min<size_t>(b->ovf.cap, 0xFFFFFFF0ull);
     p.heavy = nullptr;  // batches keep their tile order: most of their tiles would be heavy
     p.heavy_cap = 0;
-    p.busy = nullptr;
+    p.busy = b->busy.ptr;
+    p.busy_map = BUSY_RANGES;
     p.count_next = nullptr;
     p.zero_n = 0;
 
@@ -653,6 +650,7 @@ euchip_status enqueue_batch(euchip_batch *b, bool capturing = false) {
     if (b->active.empty()) return EUCHIP_OK;
     const size_t words = sizeof(Counters) / 4 + b->active.size();
     HIP_TRY(ctx, launch_fill_u32(b->ctrbuf.ptr, 0u, words, b->stream));
+    HIP_TRY(ctx, launch_fill_u32(b->tile_count.ptr, 0u, b->n_tiles, b->stream));  // raster leaves them as it read them
     euchip_status s;
     if (ctx->profiling && !capturing) {
         if ((s = timed_on(ctx, b->stream, "batch_setup", [&] { return launch_setup(b->p, b->stream, true, b->setup_blocks); })))
@@ -719,7 +717,6 @@ euchip_status wait_batch(euchip_batch *b) {
         euchip_status s;
         if ((s = ensure(ctx, b->ovf, b->ovf.cap * 4, "batch overflow list", b->stream))) return s;
         b->tables_dirty = true;
-        b->counts_dirty = true;  // the stopped raster pass left its tile counters
         if ((s = launch_batch(b, false))) return s;
     }
 }
@@ -735,6 +732,7 @@ void free_batch_memory(euchip_batch *b) {
     release(ctx, b->rec1);
     release(ctx, b->vary);
     release(ctx, b->tile_count);
+    release(ctx, b->busy);
     release(ctx, b->bins);
     release(ctx, b->ctrbuf);
     release(ctx, b->ovf);
@@ -752,7 +750,7 @@ size_t request_scratch_estimate(const euchip_pipeline_desc *d, const euchip_mesh
     const uint64_t n_sv = (uint64_t)m->n_instances * m->n_vertices;
     const uint64_t nt = (uint64_t)((W + 31) / 32) * ((H + 7) / 8);
     const uint64_t nb = (std::max<uint64_t>(n_sv, n_tris) + 255) / 256;
-    return sizeof(View) + nb * sizeof(uint2) + nt * 8 + n_sv * 4 * vd_count(d->shader) +
+    return sizeof(View) + nb * sizeof(uint2) + nt * 12 + n_sv * 4 * vd_count(d->shader) +
            n_tris * 16 * (kRec0Quads + kRec1Quads) + nt * initial_cap(n_tris, nt) * 4 + 4 + 8 * n_tris;
 }
 
@@ -1158,7 +1156,7 @@ size_t euchip_batch_scratch_bytes(const euchip_batch *b) {
     if (!b) return 0;
     return b->tables.cap * 16 + b->rec0.cap * 16 +
            b->rec1.cap * 16 +
-           b->vary.cap * 4 + b->tile_count.cap * 4 + b->bins.cap * 4 + b->ctrbuf.cap * 4 + b->ovf.cap * 8;
+           b->vary.cap * 4 + b->tile_count.cap * 4 + b->busy.cap * 4 + b->bins.cap * 4 + b->ctrbuf.cap * 4 + b->ovf.cap * 8;
 }
 
 size_t euchip_batch_estimate_bytes(const euchip_pipeline_desc *desc, const euchip_request *reqs, uint32_t n) {
