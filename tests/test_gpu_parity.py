@@ -49,6 +49,34 @@ def test_plain_cull_coords(gpu_ctx, plain, cull, cm):
     assert_bit_exact(r, f"{cull.name} {cm}")
 
 
+def test_async_frames_changing_sizes(plain):
+    """Back-to-back frames without a host sync in between, on a fresh context,
+    with the tile count growing and shrinking from frame to frame: every frame
+    bins into counts the previous frames left zeroed (two alternating halves,
+    each re-zeroed by the frame after the one that filled it)."""
+    from euc_amd.renderer import Context
+    from oracle import oracle as O
+    sizes = [(1920, 1080), (64, 48), (1, 1), (2048, 64), (640, 480), (1920, 1080), (33, 900), (640, 480)]
+    desc = flat()
+    ctx = Context(0)
+    try:
+        gm = ctx.mesh(plain.mesh)
+        outs = []
+        for w, h in sizes:
+            c, d = ctx.color_target(w, h), ctx.depth_target(w, h)
+            ctx.render(desc, plain.uniforms, gm, c, d, clear_color=0x01020304, clear_depth=1.0, sync=False)
+            outs.append((c, d))
+        ctx.synchronize()
+        for (w, h), (c, d) in zip(sizes, outs):
+            oc = np.full((h, w), 0x01020304, np.uint32)
+            od = np.full((h, w), 1.0, np.float32)
+            rc, _ = O.render(desc, plain.uniforms, plain.mesh, oc, od)
+            assert_bit_exact({"o_rc": rc, "g_rc": 0, "o_color": oc, "o_depth": od, "g_color": c.download(),
+                              "g_depth": d.download()}, f"async frame {w}x{h}")
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("wh", [(160, 120), (141, 141), (317, 211), (1000, 37), (33, 900), (1, 1), (2048, 64)])
 @pytest.mark.parametrize("bands", [BandPolicy.PAR, BandPolicy.SEQ])
 def test_plain_odd_sizes(gpu_ctx, plain, wh, bands):
