@@ -852,8 +852,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     if (blockIdx.x < 8u && tid == 0u) p.ctr->next[npar][xcd][0] = 0u;  // next frame's queue
 
     // One tile with `cnt` bin entries: winner keys in LDS, then the resolve.
+    // With `claim`, thread 0 takes the next queue index when the keys are done
+    // (the atomic's latency hides behind the resolve) and leaves it in sh.next.
     // Ends with a barrier (LDS is reused by the next tile).
-    auto run_tile = [&](uint32_t tile, uint32_t cnt) {
+    auto run_tile = [&](uint32_t tile, uint32_t cnt, bool claim) {
         uint32_t r = 0;
         if constexpr (BATCH) r = __builtin_amdgcn_readfirstlane(p.tile_req[tile]);
         const View &v = view_of<BATCH>(p, r);
@@ -895,6 +897,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             }
             raster_batch(p, v, sh, tid, valid, prim, g, mode);
         }
+        uint32_t claimed = 0u;
+        if (claim && tid == 0u) claimed = per_xcd + atomicAdd(next, 1u);
         if (inside) {
             const uint64_t kk = sh.key[tid];
             bool has;
@@ -926,6 +930,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
                 else if (v.clear_depth_en) store_stream(&v.depth[pidx], v.clear_depth);
             }
         }
+        if (claim && tid == 0u) sh.next = claimed;
         __syncthreads();
     };
 
@@ -939,16 +944,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     //    start early instead of ending the launch late).
     for (uint32_t k = blockIdx.x; k < nh; k += gridDim.x) {
         const uint32_t tile = __builtin_amdgcn_readfirstlane(p.heavy[k]);
-        run_tile(tile, __builtin_amdgcn_readfirstlane(p.tile_count[tile]));
+        run_tile(tile, __builtin_amdgcn_readfirstlane(p.tile_count[tile]), false);
     }
     // 2. Busy tiles of this XCD's list: the first by workgroup index, the rest
     //    from the XCD's counter (heavy ones are done).
     for (uint32_t k = w; k < nbusy;) {
         const uint32_t tile = k == w ? first : __builtin_amdgcn_readfirstlane(list[k]);
         const uint32_t cnt = k == w ? first_cnt : __builtin_amdgcn_readfirstlane(p.tile_count[tile]);
-        if (!(p.heavy_cap && cnt >= kHeavyTile)) run_tile(tile, cnt);
-        if (tid == 0u) sh.next = per_xcd + atomicAdd(next, 1u);
-        __syncthreads();
+        if (!(p.heavy_cap && cnt >= kHeavyTile)) {
+            run_tile(tile, cnt, true);
+        } else {
+            if (tid == 0u) sh.next = per_xcd + atomicAdd(next, 1u);
+            __syncthreads();
+        }
         k = __builtin_amdgcn_readfirstlane(sh.next);
         __syncthreads();
     }
