@@ -695,6 +695,25 @@ struct TileGeom {
     uint32_t X0, Y0, sx, TW, TH;
 };
 
+// The tile's pixels [xs, xe) of one row (index `row` of the first), w_hom at xs
+// given: each pixel's fragment updates its winner key.  Specialised by key mode
+// so the write modes' loops hold no LDS read (nothing then waits on the
+// atomics of earlier pixels).
+template <uint32_t MODE>
+EUCHIP_DEV void walk_row(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t xs, uint32_t xe, uint32_t pi0,
+                         float w0, float w1, float w2, uint64_t *key, const float *d0s) {
+    for (uint32_t x = xs; x < xe; x++) {
+        float z;
+        if (frag_z(p, R, w0, w1, w2, z)) {
+            const uint32_t pi = pi0 + (x - xs);
+            update_key(key + pi, MODE, p.depth_test, z, MODE == 0u ? d0s[pi] : 0.0f, prim);
+        }
+        w0 = w0 + R.dx[0];
+        w1 = w1 + R.dx[1];
+        w2 = w2 + R.dx[2];
+    }
+}
+
 // One row y of a triangle inside the tile: the row span (triangles.rs:219-254),
 // then the walk from the row start exactly as the reference does
 // (triangles.rs:257-301), testing the tile's pixels of the span.
@@ -715,17 +734,10 @@ EUCHIP_DEV void raster_row(const FrameParams &p, const TriRec &R, uint32_t prim,
         w1 = w1 + R.dx[1];
         w2 = w2 + R.dx[2];
     }
-    const uint32_t row = r << g.sx;
-    for (uint32_t x = xs; x < xe; x++) {
-        float z;
-        if (frag_z(p, R, w0, w1, w2, z)) {
-            const uint32_t pi = row + (x - g.X0);
-            update_key(key + pi, mode, p.depth_test, z, d0s[pi], prim);
-        }
-        w0 = w0 + R.dx[0];
-        w1 = w1 + R.dx[1];
-        w2 = w2 + R.dx[2];
-    }
+    const uint32_t pi0 = (r << g.sx) + (xs - g.X0);
+    if (mode == 1u) walk_row<1u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
+    else if (mode == 2u) walk_row<2u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
+    else walk_row<0u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
 }
 
 constexpr uint32_t kMaxTileH = 32;
