@@ -754,6 +754,7 @@ struct TileShared {
     float d0s[256];                 // depth before the pass (the test's reference)
     TriLds tri[kBatch];             // the batch's triangles
     uint32_t rows_incl[256];        // inclusive prefix sum of their row counts in the tile
+    uint8_t owner[256];             // entry of each (entry, row) item of the current 256
     uint32_t wave_rows[4];          // row counts per wave (prefix-sum carry)
     uint32_t band[2 * kMaxTileH];   // [by0, by1) of the render_par band of each row
     uint32_t next;                  // tile taken from the dynamic queue
@@ -797,22 +798,35 @@ EUCHIP_DEV uint32_t block_inclusive_sum(TileShared &sh, uint32_t tid, uint32_t v
 // One batch of up to 256 bin entries: each lane stages its entry's setup in LDS,
 // then the (entry, row) pairs of the batch are spread over all lanes so no
 // lane walks much more than one row.
+template <bool OWNER>
 EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh, uint32_t tid, bool valid,
                              uint32_t prim, const TileGeom &g, uint32_t mode) {
     const uint32_t rows = valid ? stage_entry(p, v, sh, tid, prim, g) : 0u;
     const uint32_t incl = block_inclusive_sum(sh, tid, rows);
+    const uint32_t excl = incl - rows;
     sh.rows_incl[tid] = incl;
+    // items [excl, incl) are this entry's rows; the owner table maps the first
+    // 256 items back to their entries with one LDS read, later ones (batches
+    // past 256 rows) search the prefix sums
+    if constexpr (OWNER)
+        for (uint32_t k = excl; k < incl && k < 256u; k++) sh.owner[k] = (uint8_t)tid;
     __syncthreads();
     const uint32_t total = sh.rows_incl[255];  // lanes past the batch add nothing
     for (uint32_t k = tid; k < total; k += 256u) {
-        uint32_t lo = 0, hi = kBatch - 1u;  // first entry whose inclusive sum exceeds k
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (sh.rows_incl[mid] > k) hi = mid;
-            else lo = mid + 1u;
+        uint32_t e;
+        if (OWNER && k < 256u) {
+            e = sh.owner[k];
+        } else {
+            uint32_t lo = 0, hi = kBatch - 1u;  // first entry whose inclusive sum exceeds k
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sh.rows_incl[mid] > k) hi = mid;
+                else lo = mid + 1u;
+            }
+            e = lo;
         }
-        const uint32_t base = lo ? sh.rows_incl[lo - 1u] : 0u;
-        const TriLds &T = sh.tri[lo];
+        const uint32_t base = e ? sh.rows_incl[e - 1u] : 0u;
+        const TriLds &T = sh.tri[e];
         TriRec R;
 #pragma unroll
         for (int i = 0; i < 3; i++) R.o[i] = T.o[i], R.dx[i] = T.dx[i], R.dy[i] = T.dy[i], R.hz[i] = T.hz[i];
@@ -907,7 +921,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
                 const uint2 e = p.ovf[i - nb];
                 valid = e.x == tile, prim = e.y;
             }
-            raster_batch(p, v, sh, tid, valid, prim, g, mode);
+            raster_batch<!BATCH>(p, v, sh, tid, valid, prim, g, mode);
         }
         uint32_t claimed = 0u;
         if (claim && tid == 0u) claimed = per_xcd + atomicAdd(next, 1u);
