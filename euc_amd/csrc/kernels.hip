@@ -87,21 +87,52 @@ EUCHIP_DEV void for_each_tile(bool act, uint32_t tx0, uint32_t ty0, uint32_t nx,
 // ---------------------------------------------------------------------------
 EUCHIP_DEV bool tri_setup(const FrameParams &p, const View &v, uint32_t t, TriRec &R) {
     float hom[3][4], euc[3][3];
-    uint32_t sid[3];
+    uint32_t sid[3], inst[3], vi[3];
     const float flip1 = p.y_up ? -1.0f : 1.0f;
+    // vertex_clip of the three corners, its uniform branches taken outside the
+    // corner loop so the loads of all three corners are in flight together
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const uint32_t e = 3u * t + (uint32_t)i;
-        const uint32_t inst = e / v.count;
-        const uint32_t k = e - inst * v.count;
-        const uint32_t vi = v.idx ? v.idx[k] : k;
-        sid[i] = v.sv_base + inst * v.n_vertices + vi;
-        float c[4];
-        vertex_clip(p, v, inst, vi, c);
-        hom[i][0] = c[0] * 1.0f;
-        hom[i][1] = c[1] * flip1;
-        hom[i][2] = c[2];
-        hom[i][3] = c[3];
+        inst[i] = e / v.count;
+        vi[i] = e - inst[i] * v.count;
+    }
+    if (v.idx) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) vi[i] = v.idx[vi[i]];
+    }
+    float c[3][4];
+    if (p.xform == 0u) {
+        if (v.pos_comps == 4u) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const float *P = v.pos + 4u * vi[i];
+                c[i][0] = P[0], c[i][1] = P[1], c[i][2] = P[2], c[i][3] = P[3];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const float *P = v.pos + (size_t)v.pos_comps * vi[i];
+                c[i][0] = P[0], c[i][1] = P[1], c[i][2] = P[2], c[i][3] = 1.0f;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float *P = v.pos + (size_t)v.pos_comps * vi[i];
+            const float p4[4] = {P[0], P[1], P[2], 1.0f};
+            float w[4];
+            mat4_vec4(v.models + 16u * inst[i], p4, w);
+            mat4_vec4(v.u + U_VP, w, c[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        sid[i] = v.sv_base + inst[i] * v.n_vertices + vi[i];
+        hom[i][0] = c[i][0] * 1.0f;
+        hom[i][1] = c[i][1] * flip1;
+        hom[i][2] = c[i][2];
+        hom[i][3] = c[i][3];
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) {
