@@ -874,7 +874,7 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
 // spilled registers with the light shaders; the shadow shaders would spill too
 // much and keep the default.
 constexpr uint32_t raster_waves(uint32_t sh) {
-    return (sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 1u : sh == TEXTURED ? 6u : 7u;
+    return (sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 1u : sh == TEXTURED ? 6u : (sh == DEPTH_ONLY || sh == FLAT) ? 8u : 7u;
 }
 
 template <uint32_t SH, bool BATCH>
