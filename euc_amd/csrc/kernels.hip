@@ -406,7 +406,7 @@ EUCHIP_DEV bool covers_any(const FrameParams &p, const TriRec &R) {
 }
 
 constexpr uint32_t kHash = 1024;   // LDS hash slots (distinct tiles) per setup workgroup
-constexpr uint32_t kPairs = 2048;  // (tile, primitive) pairs grouped in LDS per workgroup
+constexpr uint32_t kPairs = 1024;  // (tile, primitive) pairs grouped in LDS per workgroup
 
 // ---------------------------------------------------------------------------
 // k_setup: per-vertex clip position + VertexData (thread t = shaded vertex t)
