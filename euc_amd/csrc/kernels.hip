@@ -405,7 +405,7 @@ EUCHIP_DEV bool covers_any(const FrameParams &p, const TriRec &R) {
     return false;
 }
 
-constexpr uint32_t kHash = 1024;   // LDS hash slots (distinct tiles) per setup workgroup
+constexpr uint32_t kHash = 512;    // LDS hash slots (distinct tiles) per setup workgroup
 constexpr uint32_t kPairs = 1024;  // (tile, primitive) pairs grouped in LDS per workgroup
 
 // ---------------------------------------------------------------------------
@@ -419,7 +419,6 @@ constexpr uint32_t kPairs = 1024;  // (tile, primitive) pairs grouped in LDS per
 union SetupShared {
     struct {
         uint4 q0[256 * kRec0Quads];
-        uint4 q1[256 * kRec1Quads];
         uint32_t kept[256];
     } rec;
     struct {
@@ -429,7 +428,7 @@ union SetupShared {
 };
 
 template <uint32_t SH, bool BATCH>
-__global__ __launch_bounds__(256) void k_setup(FrameParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_setup(FrameParams p) {
     __shared__ SetupShared sm;
     __shared__ uint32_t npl;
     __shared__ uint32_t lcnt[9], lbase[9];  // new busy tiles per list (8) and new heavy tiles (8)
@@ -458,8 +457,10 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
             pack_rec(R, q0, q1);
 #pragma unroll
             for (uint32_t k = 0; k < kRec0Quads; k++) sm.rec.q0[tid * kRec0Quads + k] = q0[k];
+            // rec1: this lane's 32 bytes, contiguous with its neighbours' (no staging)
 #pragma unroll
-            for (uint32_t k = 0; k < kRec1Quads; k++) sm.rec.q1[tid * kRec1Quads + k] = q1[k];
+            for (uint32_t k = 0; k < kRec1Quads; k++)
+                store_stream(&p.rec1[((size_t)v.tri_base + t) * kRec1Quads + k], q1[k]);
         }
     }
     sm.rec.kept[tid] = keep ? 1u : 0u;
@@ -486,11 +487,6 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
             const uint32_t q = k * 256u + tid;
             if (sm.rec.kept[q / kRec0Quads]) store_stream(&p.rec0[base * kRec0Quads + q], sm.rec.q0[q]);
         }
-#pragma unroll
-        for (uint32_t k = 0; k < kRec1Quads; k++) {
-            const uint32_t q = k * 256u + tid;
-            if (sm.rec.kept[q / kRec1Quads]) store_stream(&p.rec1[base * kRec1Quads + q], sm.rec.q1[q]);
-        }
     }
     __syncthreads();  // the binning tables reuse the record staging
     for (uint32_t i = tid; i < kHash; i += 256u) sm.bin.hkey[i] = 0u, sm.bin.hcnt[i] = 0u;
@@ -499,7 +495,7 @@ __global__ __launch_bounds__(256) void k_setup(FrameParams p) {
     const uint32_t tbase = v.tile_base;
     for_each_tile(keep, tx0, ty0, nx, ny, v.ntx, t, 0u, [&](uint32_t lt, uint32_t, uint32_t prim, uint32_t) {
         const uint32_t tile = tbase + lt;
-        uint32_t h = (tile * 2654435761u) >> 22;  // 10-bit multiplicative hash
+        uint32_t h = (tile * 2654435761u) >> 23;  // 9-bit multiplicative hash
         for (uint32_t probe = 0; probe < 16u; probe++) {
             const uint32_t old = atomicCAS(&sm.bin.hkey[h], 0u, tile + 1u);
             if (old == 0u || old == tile + 1u) {
