@@ -866,11 +866,10 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
     __syncthreads();  // the next batch reuses sh.tri and sh.rows_incl
 }
 
-// Waves per SIMD the raster pass is compiled for: latency hiding pays for a few
-// spilled registers with the light shaders; the shadow shaders would spill too
-// much and keep the default.
+// Waves per SIMD the raster pass is compiled for (measured per shader): latency
+// hiding pays for a few spilled registers, up to a point.
 constexpr uint32_t raster_waves(uint32_t sh) {
-    return (sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 1u : sh == TEXTURED ? 6u : (sh == DEPTH_ONLY || sh == FLAT) ? 8u : 7u;
+    return (sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 5u : sh == TEXTURED ? 7u : (sh == DEPTH_ONLY || sh == FLAT) ? 8u : 7u;
 }
 
 template <uint32_t SH, bool BATCH>
