@@ -56,6 +56,7 @@ struct Counters {
     uint32_t ovf[2][32];
     uint32_t next[2][8][32];  // per frame parity and XCD: next tile of the dynamic tile loop
     uint32_t heavy[2][32];    // per frame parity: entries of the heavy-tile list
+    uint32_t big[2][32];      // per frame parity: entries of the big-triangle list
     uint32_t busy[2][8][32];  // per frame parity and XCD: entries of the busy-tile list
     uint32_t flags;      // FrameFlags, sticky until the host clears them
     uint32_t max_count;  // largest per-tile triangle count seen by a one-request frame
@@ -72,6 +73,15 @@ constexpr uint32_t kHeavyTile = 160u;
 // of 4 x 4 tiles dealt over the XCDs (one-request frames: each XCD gets a share
 // of every screen region, so dense regions do not load one XCD).
 enum BusyMap : uint32_t { BUSY_RANGES = 0, BUSY_BLOCKS = 1 };
+
+// One-request frames of meshes with no more triangles than the target has tiles
+// ("big-triangle mode"): a triangle whose box covers more tiles than this goes
+// to the big-triangle list instead of every tile's bin (while the list has
+// room), and every tile of the target scans that list (a raster kernel of its
+// own).  Binning such a triangle costs one global atomic per tile when the
+// workgroup's LDS grouping overflows.
+constexpr uint32_t kBigTiles = 64;
+constexpr uint32_t kBigCap = 4096;
 
 // One render request: a target pair, a mesh, a uniform block, and where the
 // request's vertices, tiles and bins sit in the launch's scratch memory.
@@ -132,6 +142,8 @@ struct FrameParams {
     // buffer; batches zero their counts with a fill before setup).
     uint32_t *busy;
     uint32_t busy_map;
+    uint32_t *big;         // big triangles (primitive ids), big_cap slots; big_cap 0: off (batches)
+    uint32_t big_cap;
     uint32_t *count_next;
     uint32_t zero_n;
     uint32_t frame;        // frame parity selects the overflow counter and tile queue

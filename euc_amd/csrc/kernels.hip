@@ -453,6 +453,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
             ty0 = R.ymin >> sy;
             nx = ((R.xmax - 1u) >> sx) - tx0 + 1u;
             ny = ((R.ymax - 1u) >> sy) - ty0 + 1u;
+            if (p.big_cap && nx * ny > kBigTiles) {  // on the big-triangle list instead of the bins
+                const uint32_t k = atomicAdd(&p.ctr->big[p.frame & 1u][0], 1u);
+                if (k < p.big_cap) p.big[k] = t, nx = ny = 0u;
+            }
             uint4 q0[kRec0Quads], q1[kRec1Quads];
             pack_rec(R, q0, q1);
 #pragma unroll
@@ -872,7 +876,7 @@ constexpr uint32_t raster_waves(uint32_t sh) {
     return (sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 5u : sh == TEXTURED ? 7u : (sh == DEPTH_ONLY || sh == FLAT) ? 8u : 7u;
 }
 
-template <uint32_t SH, bool BATCH>
+template <uint32_t SH, bool BATCH, bool BIG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(SH)))) void k_raster(FrameParams p) {
     __shared__ TileShared sh;
     const uint32_t tid = threadIdx.x;
@@ -890,6 +894,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     const uint32_t flags = p.ctr->flags;
     const uint32_t novf0 = p.ctr->ovf[par][0];
     uint32_t nh = __builtin_amdgcn_readfirstlane(p.ctr->heavy[par][0]);
+    uint32_t nbig = BIG ? __builtin_amdgcn_readfirstlane(p.ctr->big[par][0]) : 0u;
     uint32_t nbusy = __builtin_amdgcn_readfirstlane(p.ctr->busy[par][xcd][0]);
     const uint32_t first = w < lcap ? __builtin_amdgcn_readfirstlane(list[w]) : 0u;
     const uint32_t first_cnt = __builtin_amdgcn_readfirstlane(p.tile_count[first < ntiles ? first : 0u]);
@@ -933,9 +938,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
         if (tid == 0u && cnt > cap) atomicMax(p.req_max + r, cnt);
         __syncthreads();
-        // this tile's bin, then (only when it overflowed) its entries of the overflow list
+        // this tile's bin, then (only when it overflowed) its entries of the
+        // overflow list, then the big triangles (staging drops those that miss it)
         const uint32_t nb = cnt < cap ? cnt : cap;
-        const uint32_t nscan = nb + (cnt > cap ? novf : 0u);
+        const uint32_t no = cnt > cap ? novf : 0u;
+        const uint32_t nscan = nb + no + nbig;
         for (uint32_t b = 0; b < nscan; b += kBatch) {
             const uint32_t i = b + tid;
             bool valid = false;
@@ -943,9 +950,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             if (tid >= kBatch || i >= nscan) {
             } else if (i < nb) {
                 valid = true, prim = bin[i];
-            } else {
+            } else if (i < nb + no) {
                 const uint2 e = p.ovf[i - nb];
                 valid = e.x == tile, prim = e.y;
+            } else {
+                valid = BIG, prim = BIG ? p.big[i - nb - no] : 0u;
             }
             raster_batch<!BATCH>(p, v, sh, tid, valid, prim, g, mode);
         }
@@ -989,7 +998,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     // The next frame's counts start at zero (stores only, issued first).
     for (uint32_t i = blockIdx.x * 256u + tid; i < p.zero_n; i += gridDim.x * 256u) p.count_next[i] = 0u;
     if (blockIdx.x == 0u && tid == 0u) p.ctr->heavy[npar][0] = 0u;
+    if (BIG && blockIdx.x == 0u && tid == 0u) p.ctr->big[npar][0] = 0u;
     if (blockIdx.x < 8u && tid == 0u) p.ctr->busy[npar][xcd][0] = 0u;
+    if constexpr (BIG) nbig = nbig < p.big_cap ? nbig : p.big_cap;
+    // With big triangles every tile may hold fragments: the loop below then takes
+    // every tile of this XCD's contiguous range instead of its busy list.
+    if (BIG && nbig) {
+        uint32_t b0, b1;
+        xcd_range(ntiles, xcd, b0, b1);
+        nbusy = b1 - b0;
+    }
     nh = nh < p.heavy_cap ? nh : p.heavy_cap;
     nbusy = nbusy < lcap ? nbusy : lcap;
     // 1. Heavy tiles first, workgroup k taking heavy[k] (the longest tiles
@@ -1001,8 +1019,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     // 2. Busy tiles of this XCD's list: the first by workgroup index, the rest
     //    from the XCD's counter (heavy ones are done).
     for (uint32_t k = w; k < nbusy;) {
-        const uint32_t tile = k == w ? first : __builtin_amdgcn_readfirstlane(list[k]);
-        const uint32_t cnt = k == w ? first_cnt : __builtin_amdgcn_readfirstlane(p.tile_count[tile]);
+        uint32_t tile;
+        if (BIG && nbig) {
+            uint32_t b0, b1;
+            xcd_range(ntiles, xcd, b0, b1);
+            tile = b0 + k;
+        } else {
+            tile = k == w ? first : __builtin_amdgcn_readfirstlane(list[k]);
+        }
+        const uint32_t cnt = k == w && !nbig ? first_cnt : __builtin_amdgcn_readfirstlane(p.tile_count[tile]);
         if (!(p.heavy_cap && cnt >= kHeavyTile)) {
             run_tile(tile, cnt, true);
         } else {
@@ -1014,7 +1039,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     }
     // 3. Empty tiles of this workgroup's share (every per_xcd-th tile of the
     //    XCD's contiguous range): the clears, stores only, no LDS.
-    if (!BATCH && !p.v0.clear_color_en && !p.v0.clear_depth_en) return;
+    if ((BIG && nbig) || (!BATCH && !p.v0.clear_color_en && !p.v0.clear_depth_en)) return;
     uint32_t t0, t1;
     xcd_range(ntiles, xcd, t0, t1);
     for (uint32_t base = t0 + w; base < t1; base += 8u * per_xcd) {
@@ -1082,12 +1107,12 @@ hipError_t launch_setup(const FrameParams &p, hipStream_t s, bool batched, uint3
 // Workgroups of a raster kernel that fit on the device at once (`cus` compute
 // units); the busy-tile loop gives every workgroup a first tile, so workgroups
 // that could only start when others finish would hold theirs back.
-template <uint32_t SH, bool BATCH>
+template <uint32_t SH, bool BATCH, bool BIG>
 static uint32_t resident_groups(uint32_t cus) {
     static int per_cu = 0;
     if (per_cu == 0) {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_raster<SH, BATCH>), 256, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_raster<SH, BATCH, BIG>), 256, 0) !=
                 hipSuccess || n < 1)
             n = 8;
         per_cu = n;
@@ -1098,11 +1123,14 @@ static uint32_t resident_groups(uint32_t cus) {
 template <uint32_t SH>
 static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, uint32_t groups, uint32_t cus) {
     if (batched) {
-        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, true>(cus) / 8u * 8u));
-        hipLaunchKernelGGL((k_raster<SH, true>), dim3(groups), dim3(256), 0, s, p);
+        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, true, false>(cus) / 8u * 8u));
+        hipLaunchKernelGGL((k_raster<SH, true, false>), dim3(groups), dim3(256), 0, s, p);
+    } else if (p.big_cap) {
+        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, false, true>(cus) / 8u * 8u));
+        hipLaunchKernelGGL((k_raster<SH, false, true>), dim3(groups), dim3(256), 0, s, p);
     } else {
-        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, false>(cus) / 8u * 8u));
-        hipLaunchKernelGGL((k_raster<SH, false>), dim3(groups), dim3(256), 0, s, p);
+        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, false, false>(cus) / 8u * 8u));
+        hipLaunchKernelGGL((k_raster<SH, false, false>), dim3(groups), dim3(256), 0, s, p);
     }
 }
 

@@ -56,6 +56,7 @@ struct euchip_ctx {
     DevArray<uint2> ovf;
     DevArray<uint32_t> heavy;   // heavy-tile list
     DevArray<uint32_t> busy;    // busy-tile lists (one per XCD tile range)
+    DevArray<uint32_t> big;     // big-triangle list
     uint32_t counts_used[2] = {0, 0};  // tiles of the last frame that counted into each half of tile_count
     uint32_t tile_cap = 128;    // bin slots per tile; raised when a frame needed more
     uint32_t frame = 0;
@@ -443,6 +444,8 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         return s;
     if ((s = ensure(ctx, ctx->heavy, ntiles, "heavy tiles", ctx->stream))) return s;
     if ((s = ensure(ctx, ctx->busy, 8 * ntiles, "busy tiles", ctx->stream))) return s;
+    const size_t big_cap = std::min<size_t>(std::max<size_t>(P.n_tris, 1), kBigCap);
+    if ((s = ensure(ctx, ctx->big, big_cap, "big triangles", ctx->stream))) return s;
 
     for (int attempt = 0;; attempt++) {
         if ((s = ensure(ctx, ctx->bins, ntiles * ctx->tile_cap, "bins", ctx->stream))) return s;
@@ -457,6 +460,10 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.zero_n = ctx->counts_used[npar];
         p.busy = ctx->busy.ptr;
         p.busy_map = BUSY_BLOCKS;
+        // big-triangle mode when the mesh has no more triangles than the target
+        // has tiles (triangles then tend to cover many tiles each)
+        p.big = ctx->big.ptr;
+        p.big_cap = P.n_tris <= ntiles ? (uint32_t)std::min<size_t>(ctx->big.cap, kBigCap) : 0u;
         p.bins = ctx->bins.ptr;
         p.v0.tile_cap = ctx->tile_cap;
         p.v0.bin_base = 0;
@@ -616,6 +623,8 @@ euchip_status build_tables(euchip_batch *b) {
     p.heavy_cap = 0;
     p.busy = b->busy.ptr;
     p.busy_map = BUSY_RANGES;
+    p.big = nullptr;
+    p.big_cap = 0;
     p.count_next = nullptr;
     p.zero_n = 0;
 
@@ -815,6 +824,7 @@ euchip_status euchip_ctx_destroy(euchip_ctx *ctx) {
     release(ctx, ctx->ovf);
     release(ctx, ctx->heavy);
     release(ctx, ctx->busy);
+    release(ctx, ctx->big);
 
     (void)hipFree(ctx->counters);
     if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);

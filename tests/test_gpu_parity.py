@@ -77,6 +77,32 @@ def test_async_frames_changing_sizes(plain):
         ctx.close()
 
 
+def test_big_triangle_list_full():
+    """No more triangles than tiles: triangles covering more than 64 tiles go to
+    the big-triangle list instead of the bins, up to 4096 of them; the rest of the
+    big ones are binned as usual.  6000 triangles at 1600x960 (6000 tiles of 32x8),
+    about 5250 of them with boxes past 64 tiles, overflow the list."""
+    from euc_amd.renderer import Context
+    rng = np.random.default_rng(0xB16)
+    n = 6000
+    c = rng.uniform(-1.0, 1.0, (n, 1, 2))
+    off = rng.uniform(-0.35, 0.35, (n, 3, 2))
+    pos = np.empty((n * 3, 4), np.float32)
+    pos[:, :2] = (c + off).reshape(-1, 2)
+    pos[:, 2] = rng.uniform(0.0, 1.0, n * 3)
+    pos[:, 3] = 1.0
+    col = (rng.integers(0, 1 << 24, n, dtype=np.uint64) | np.uint64(0xFF000000)).astype(np.uint32)
+    mesh = S.Mesh(pos=pos, count=n * 3, rgba8=np.repeat(col, 3))
+    desc = flat(cull=CullMode.NONE)
+    ctx = Context(0)
+    try:
+        r = run_pair(ctx, desc, S.uniforms(), mesh, 1600, 960)
+        assert r["frags"] > 1_000_000
+        assert_bit_exact(r, "big-triangle list past its capacity")
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("wh", [(160, 120), (141, 141), (317, 211), (1000, 37), (33, 900), (1, 1), (2048, 64)])
 @pytest.mark.parametrize("bands", [BandPolicy.PAR, BandPolicy.SEQ])
 def test_plain_odd_sizes(gpu_ctx, plain, wh, bands):
