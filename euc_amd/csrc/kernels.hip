@@ -89,8 +89,9 @@ EUCHIP_DEV bool tri_setup(const FrameParams &p, const View &v, uint32_t t, TriRe
     float hom[3][4], euc[3][3];
     uint32_t sid[3], inst[3], vi[3];
     const float flip1 = p.y_up ? -1.0f : 1.0f;
-    // vertex_clip of the three corners, its uniform branches taken outside the
-    // corner loop so the loads of all three corners are in flight together
+    // the corners' clip positions (vertex(): VP * model * (p, 1), or the CLIP
+    // input itself), the uniform branches taken outside the corner loop so the
+    // loads of all three corners are in flight together
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const uint32_t e = 3u * t + (uint32_t)i;

@@ -143,17 +143,6 @@ EUCHIP_DEV void vertex_world(const FrameParams &p, const View &v, uint32_t inst,
     }
 }
 
-// Homogeneous clip position returned by vertex().
-EUCHIP_DEV void vertex_clip(const FrameParams &p, const View &v, uint32_t inst, uint32_t vi, float clip[4]) {
-    float w[4];
-    vertex_world(p, v, inst, vi, w);
-    if (p.xform == 0) {
-        clip[0] = w[0], clip[1] = w[1], clip[2] = w[2], clip[3] = w[3];
-    } else {
-        mat4_vec4(v.u + U_VP, w, clip);
-    }
-}
-
 // VertexData returned by vertex().
 template <uint32_t SH>
 EUCHIP_DEV void vertex_data(const FrameParams &p, const View &v, uint32_t inst, uint32_t vi, float *vd) {
