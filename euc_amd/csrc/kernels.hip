@@ -761,7 +761,18 @@ EUCHIP_DEV void raster_row(const FrameParams &p, const TriRec &R, uint32_t prim,
     float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
     float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
     float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
-    for (uint32_t x = x0; x < xs; x++) {
+    // to the tile's first pixel, one addition per pixel as the reference walks
+    // (four pixels per turn: the same additions, a quarter of the loop control)
+    uint32_t x = x0;
+    for (; x + 4u <= xs; x += 4u) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            w0 = w0 + R.dx[0];
+            w1 = w1 + R.dx[1];
+            w2 = w2 + R.dx[2];
+        }
+    }
+    for (; x < xs; x++) {
         w0 = w0 + R.dx[0];
         w1 = w1 + R.dx[1];
         w2 = w2 + R.dx[2];
