@@ -1009,8 +1009,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
 
     // The next frame's counts start at zero (stores only, issued first).
     for (uint32_t i = blockIdx.x * 256u + tid; i < p.zero_n; i += gridDim.x * 256u) p.count_next[i] = 0u;
-    if (blockIdx.x == 0u && tid == 0u) p.ctr->heavy[npar][0] = 0u;
-    if (BIG && blockIdx.x == 0u && tid == 0u) p.ctr->big[npar][0] = 0u;
+    // next frame's lists start empty (every instantiation: the next frame may be
+    // in big-triangle mode when this one is not)
+    if (blockIdx.x == 0u && tid == 0u) p.ctr->heavy[npar][0] = 0u, p.ctr->big[npar][0] = 0u;
     if (blockIdx.x < 8u && tid == 0u) p.ctr->busy[npar][xcd][0] = 0u;
     if constexpr (BIG) nbig = nbig < p.big_cap ? nbig : p.big_cap;
     // With big triangles every tile may hold fragments: the loop below then takes

@@ -103,6 +103,37 @@ def test_big_triangle_list_full():
         ctx.close()
 
 
+def test_big_triangle_mode_after_other_frames(plain):
+    """Big-triangle mode, then a frame without it, then big-triangle mode on the
+    same frame parity with another mesh: the last frame must scan only its own
+    big triangles."""
+    from euc_amd.renderer import Context
+    from oracle import oracle as O
+    rng = np.random.default_rng(0xB17)
+    n = 16
+    pos = np.empty((n * 3, 4), np.float32)
+    pos[:, :2] = rng.uniform(-0.6, 0.6, (n * 3, 2))
+    pos[:, 2] = rng.uniform(0.0, 1.0, n * 3)
+    pos[:, 3] = 1.0
+    col = (rng.integers(0, 1 << 24, n, dtype=np.uint64) | np.uint64(0xFF000000)).astype(np.uint32)
+    other = S.Mesh(pos=pos, count=n * 3, rgba8=np.repeat(col, 3))
+    desc = flat(cull=CullMode.NONE)
+    ctx = Context(0)
+    try:
+        frames = [(plain.mesh, 640, 480), (plain.mesh, 64, 48), (other, 640, 480)]  # big, not big, big
+        for mesh, w, h in frames:
+            gm = ctx.mesh(mesh)
+            c, d = ctx.color_target(w, h), ctx.depth_target(w, h)
+            ctx.render(desc, S.uniforms(), gm, c, d, clear_color=0, clear_depth=1.0)
+            oc = np.zeros((h, w), np.uint32)
+            od = np.ones((h, w), np.float32)
+            rc, _ = O.render(desc, S.uniforms(), mesh, oc, od)
+            assert_bit_exact({"o_rc": rc, "g_rc": 0, "o_color": oc, "o_depth": od, "g_color": c.download(),
+                              "g_depth": d.download()}, f"{len(mesh.pos) // 3} triangles at {w}x{h}")
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("wh", [(160, 120), (141, 141), (317, 211), (1000, 37), (33, 900), (1, 1), (2048, 64)])
 @pytest.mark.parametrize("bands", [BandPolicy.PAR, BandPolicy.SEQ])
 def test_plain_odd_sizes(gpu_ctx, plain, wh, bands):
