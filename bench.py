@@ -566,8 +566,8 @@ def bench_serving(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              f"(the path would shard screen tiles across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this renderer runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(several GPUs would each own a set of screen-tile rows)", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe()

@@ -16,8 +16,8 @@
  *   - euchip_render() blocks until the targets hold the result, like the
  *     reference's render().  The *_async variants only enqueue on the
  *     context's stream; euchip_synchronize() waits and reports deferred errors.
- *   - Errors are status codes.  The reference panics in these cases; the Rust
- *     wrapper re-panics with the reference's message.
+ *   - Errors are status codes.  The reference panics in these cases; a Rust
+ *     wrapper over this header re-panics with the reference's message.
  */
 #ifndef EUCHIP_H
 #define EUCHIP_H
