@@ -749,6 +749,7 @@ EUCHIP_DEV void walk_row(const FrameParams &p, const TriRec &R, uint32_t prim, u
 // One row y of a triangle inside the tile: the row span (triangles.rs:219-254),
 // then the walk from the row start exactly as the reference does
 // (triangles.rs:257-301), testing the tile's pixels of the span.
+template <uint32_t SKIP_UNROLL>
 EUCHIP_DEV void raster_row(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t y, const TileGeom &g,
                            uint32_t mode, uint64_t *key, const float *d0s, const uint32_t *band) {
     uint32_t x0, x1;
@@ -762,11 +763,11 @@ EUCHIP_DEV void raster_row(const FrameParams &p, const TriRec &R, uint32_t prim,
     float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
     float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
     // to the tile's first pixel, one addition per pixel as the reference walks
-    // (four pixels per turn: the same additions, a quarter of the loop control)
+    // (SKIP_UNROLL pixels per turn: the same additions, less loop control)
     uint32_t x = x0;
-    for (; x + 4u <= xs; x += 4u) {
+    for (; x + SKIP_UNROLL <= xs; x += SKIP_UNROLL) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (uint32_t q = 0; q < SKIP_UNROLL; q++) {
             w0 = w0 + R.dx[0];
             w1 = w1 + R.dx[1];
             w2 = w2 + R.dx[2];
@@ -841,7 +842,7 @@ EUCHIP_DEV uint32_t block_inclusive_sum(TileShared &sh, uint32_t tid, uint32_t v
 // One batch of up to 256 bin entries: each lane stages its entry's setup in LDS,
 // then the (entry, row) pairs of the batch are spread over all lanes so no
 // lane walks much more than one row.
-template <bool OWNER>
+template <bool OWNER, bool LONG_ROWS>
 EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh, uint32_t tid, bool valid,
                              uint32_t prim, const TileGeom &g, uint32_t mode) {
     const uint32_t rows = valid ? stage_entry(p, v, sh, tid, prim, g) : 0u;
@@ -877,7 +878,7 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
         for (int i = 0; i < 6; i++) R.vby[i] = T.vby[i];
         R.xmin = T.xmin, R.xmax = T.xmax, R.ymin = T.ymin, R.ymax = T.ymax, R.flags = T.flags;
         const uint32_t ry0 = R.ymin > g.Y0 ? R.ymin : g.Y0;
-        raster_row(p, R, T.prim, ry0 + (k - base), g, mode, sh.key, sh.d0s, sh.band);
+        raster_row<LONG_ROWS ? 8u : 4u>(p, R, T.prim, ry0 + (k - base), g, mode, sh.key, sh.d0s, sh.band);
     }
     __syncthreads();  // the next batch reuses sh.tri and sh.rows_incl
 }
@@ -968,7 +969,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             } else {
                 valid = BIG, prim = BIG ? p.big[i - nb - no] : 0u;
             }
-            raster_batch<!BATCH>(p, v, sh, tid, valid, prim, g, mode);
+            raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, prim, g, mode);
         }
         uint32_t claimed = 0u;
         if (claim && tid == 0u) claimed = per_xcd + atomicAdd(next, 1u);
