@@ -67,7 +67,7 @@ struct Counters {
 constexpr uint32_t kBatch = 128;
 // Tiles whose bin reaches this many entries are listed by setup and rasterised
 // first (the longest tiles start early instead of ending the launch late).
-constexpr uint32_t kHeavyTile = 160u;
+constexpr uint32_t kHeavyTile = 128u;
 // Which of the 8 busy-tile lists (one per XCD) a tile goes to: its XCD's
 // contiguous tile range (batches: each XCD gets whole requests), or its block
 // of 4 x 4 tiles dealt over the XCDs (one-request frames: each XCD gets a share
