@@ -765,6 +765,10 @@ size_t request_scratch_estimate(const euchip_pipeline_desc *d, const euchip_mesh
 
 size_t batch_fixed_bytes() { return sizeof(Counters) + 8 * ((size_t)1 << 20); }
 
+void set_last_error(euchip_ctx *ctx, const char *msg) {
+    if (ctx) ctx->last_error = msg;
+}
+
 }  // namespace euchip
 
 extern "C" {

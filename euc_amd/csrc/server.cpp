@@ -22,6 +22,7 @@ namespace euchip {
 // runtime.cpp
 size_t request_scratch_estimate(const euchip_pipeline_desc *d, const euchip_mesh *m, uint32_t w, uint32_t h);
 size_t batch_fixed_bytes();
+void set_last_error(euchip_ctx *ctx, const char *msg);
 }  // namespace euchip
 
 namespace {
@@ -297,6 +298,14 @@ euchip_status euchip_server_create(euchip_ctx *ctx, const euchip_pipeline_desc *
                                    uint32_t max_batch, uint32_t slots, euchip_server **out) {
     if (!ctx || !desc || !out || max_batch == 0 || slots == 0) return EUCHIP_INVALID_ARG;
     *out = nullptr;
+    // The server owns its targets and hands out pooled storage that still holds
+    // an earlier request's pixels and depth: every request must start from the
+    // desc's clear values, or it would see (and depth-test against) another's.
+    if (!desc->clear_color_enable || !desc->clear_depth_enable) {
+        euchip::set_last_error(ctx, "server pipelines must enable both clears (clear_color_enable, clear_depth_enable): "
+                                   "pooled targets hold earlier requests' results");
+        return EUCHIP_INVALID_ARG;
+    }
     euchip_server *s = new euchip_server();
     s->ctx = ctx;
     s->desc = *desc;

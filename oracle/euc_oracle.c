@@ -868,3 +868,45 @@ float orc_walk(float a, float d, int64_t n) {
     for (int64_t i = 0; i < n; i++) a = a + d;
     return a;
 }
+
+/* The fixed vertex and fragment shaders of one pipeline, exposed for the
+ * closed-form tests of the shader models (tests/test_oracle.py). */
+static int pipe_init(Pipe *p, const orc_desc *d, const float *u, const orc_stream *s, const orc_texture *tex,
+                     const orc_texture *shadow) {
+    p->d = d, p->u = u, p->s = s, p->tex = tex, p->shadow = shadow;
+    p->frag_kind = d->shader == ORC_SHADER_FLAT ? 1 : (d->shader == ORC_SHADER_DEPTH_ONLY ? 2 : 0);
+    switch (d->shader) {
+    case ORC_SHADER_FLAT: p->nvd = 1; break;
+    case ORC_SHADER_COLOR:
+    case ORC_SHADER_GOURAUD: p->nvd = 4; break;
+    case ORC_SHADER_PHONG: p->nvd = 6; break;
+    case ORC_SHADER_DEPTH_ONLY: p->nvd = 0; break;
+    case ORC_SHADER_TEXTURED: p->nvd = 2; break;
+    case ORC_SHADER_PHONG_SHADOW: p->nvd = 9; break;
+    case ORC_SHADER_TEX_SHADOW: p->nvd = 11; break;
+    default: return ORC_INVALID;
+    }
+    return ORC_OK;
+}
+
+int orc_vertex(const orc_desc *d, const float *u, const orc_stream *s, int32_t inst, int64_t vi, float pos[4],
+               float vd[12]) {
+    Pipe p;
+    if (pipe_init(&p, d, u, s, NULL, NULL)) return ORC_INVALID;
+    SVert v;
+    vertex_shader(&p, inst, vi, &v);
+    memcpy(pos, v.pos, sizeof(v.pos));
+    memcpy(vd, v.vd, sizeof(v.vd));
+    return p.nvd;
+}
+
+int orc_fragment(const orc_desc *d, const float *u, const orc_texture *tex, const orc_texture *shadow,
+                 const float *vd, float out[4], uint32_t old_pixel, uint32_t *blended) {
+    Pipe p;
+    if (pipe_init(&p, d, u, NULL, tex, shadow)) return ORC_INVALID;
+    Frag f;
+    fragment_shader(&p, vd, &f);
+    memcpy(out, f.c, sizeof(f.c));
+    if (blended) *blended = blend_shader(&p, old_pixel, &f);
+    return ORC_OK;
+}

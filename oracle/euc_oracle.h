@@ -112,6 +112,17 @@ float orc_walk(float a, float d, int64_t n);
  * sampler/linear.rs:30-66); out gets 4 (RGBA8 -> f32) or 1 (f32) values. */
 void orc_sample(const orc_texture *t, float u, float v, int32_t clamped, float *out);
 
+/* Pipeline::vertex of the fixed shader set for stream vertex `vi` of instance
+ * `inst`: the clip position and the VertexData (returns its float count). */
+int orc_vertex(const orc_desc *d, const float *uniforms, const orc_stream *s, int32_t inst, int64_t vi,
+               float pos[4], float vd[12]);
+
+/* Pipeline::fragment of the fixed shader set on VertexData vd (out: Rgba<f32>,
+ * FLAT: colour bits in out[0]) and Pipeline::blend onto old_pixel (blended may
+ * be NULL). */
+int orc_fragment(const orc_desc *d, const float *uniforms, const orc_texture *tex, const orc_texture *shadow,
+                 const float *vd, float out[4], uint32_t old_pixel, uint32_t *blended);
+
 /* The fixed fragment-shader math, exposed for unit tests. */
 void orc_lighting2(const float *uniforms, const float wpos[3], const float nrm[3], float out[4]);
 

@@ -75,6 +75,12 @@ def lib():
         L.orc_walk.argtypes = [C.c_float, C.c_float, C.c_int64]
         L.orc_sample.restype = None
         L.orc_sample.argtypes = [C.POINTER(OrcTexture), C.c_float, C.c_float, C.c_int32, C.c_void_p]
+        L.orc_vertex.restype = C.c_int
+        L.orc_vertex.argtypes = [C.POINTER(OrcDesc), C.c_void_p, C.POINTER(OrcStream), C.c_int32, C.c_int64,
+                                 C.c_void_p, C.c_void_p]
+        L.orc_fragment.restype = C.c_int
+        L.orc_fragment.argtypes = [C.POINTER(OrcDesc), C.c_void_p, C.POINTER(OrcTexture), C.POINTER(OrcTexture),
+                                   C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]
         L.orc_lighting2.restype = None
         L.orc_lighting2.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -107,6 +113,16 @@ def _texture(t):
     return tex, arr
 
 
+def _stream(mesh, arr):
+    pos = arr(mesh.pos, np.float32)
+    return OrcStream(
+        pos=_ptr(pos), pos_comps=pos.shape[1], nrm=_ptr(arr(mesh.nrm, np.float32)),
+        col=_ptr(arr(mesh.col, np.float32)), uv=_ptr(arr(mesh.uv, np.float32)),
+        rgba8=_ptr(arr(mesh.rgba8, np.uint32)), idx=_ptr(arr(mesh.idx, np.uint32)),
+        count=mesh.count, n_vertices=pos.shape[0], n_instances=mesh.n_instances,
+        models=_ptr(arr(mesh.models, np.float32)))
+
+
 def render(desc, uniforms, mesh, pixel=None, depth=None, texture=None, shadow=None):
     """Run the oracle in place on numpy targets.
 
@@ -126,13 +142,7 @@ def render(desc, uniforms, mesh, pixel=None, depth=None, texture=None, shadow=No
         keep.append(b)
         return b
 
-    pos = arr(mesh.pos, np.float32)
-    s = OrcStream(
-        pos=_ptr(pos), pos_comps=pos.shape[1], nrm=_ptr(arr(mesh.nrm, np.float32)),
-        col=_ptr(arr(mesh.col, np.float32)), uv=_ptr(arr(mesh.uv, np.float32)),
-        rgba8=_ptr(arr(mesh.rgba8, np.uint32)), idx=_ptr(arr(mesh.idx, np.uint32)),
-        count=mesh.count, n_vertices=pos.shape[0], n_instances=mesh.n_instances,
-        models=_ptr(arr(mesh.models, np.float32)))
+    s = _stream(mesh, arr)
     tex, ta = _texture(texture)
     sh, sa = _texture(shadow)
     if pixel is not None:
@@ -187,3 +197,41 @@ def lighting2(uniforms, wpos, nrm):
     out = np.zeros(4, dtype=np.float32)
     lib().orc_lighting2(u.ctypes.data, a.ctypes.data, b.ctypes.data, out.ctypes.data)
     return out
+
+
+def vertex(desc, uniforms, mesh, inst, vi):
+    """Pipeline::vertex of the fixed shader: (clip position float32[4], VertexData float32[n])."""
+    keep = []
+
+    def arr(a, dt):
+        if a is None:
+            return None
+        b = np.ascontiguousarray(a, dtype=dt)
+        keep.append(b)
+        return b
+
+    d = make_desc(desc)
+    u = np.ascontiguousarray(uniforms, dtype=np.float32)
+    s = _stream(mesh, arr)
+    pos = np.zeros(4, np.float32)
+    vd = np.zeros(12, np.float32)
+    n = lib().orc_vertex(C.byref(d), u.ctypes.data, C.byref(s), int(inst), int(vi), pos.ctypes.data, vd.ctypes.data)
+    del keep
+    return pos, vd[:n]
+
+
+def fragment(desc, uniforms, vd, texture=None, shadow=None, old_pixel=0):
+    """Pipeline::fragment then Pipeline::blend of the fixed shader: (Rgba<f32> float32[4], blended u32)."""
+    d = make_desc(desc)
+    u = np.ascontiguousarray(uniforms, dtype=np.float32)
+    v = np.zeros(12, np.float32)
+    v[:len(vd)] = np.asarray(vd, np.float32)
+    tex, ta = _texture(texture)
+    sh, sa = _texture(shadow)
+    out = np.zeros(4, np.float32)
+    blended = C.c_uint32(0)
+    rc = lib().orc_fragment(C.byref(d), u.ctypes.data, C.byref(tex) if tex else None, C.byref(sh) if sh else None,
+                            v.ctypes.data, out.ctypes.data, int(old_pixel), C.byref(blended))
+    assert rc == 0
+    del ta, sa
+    return out, blended.value
