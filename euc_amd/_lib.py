@@ -35,6 +35,7 @@ class PipelineDescC(C.Structure):
         ("clear_color_enable", C.c_uint32), ("clear_color", C.c_uint32),
         ("clear_depth_enable", C.c_uint32), ("clear_depth", C.c_float),
         ("texture", C.c_void_p), ("shadow", C.c_void_p),
+        ("pixel_format", C.c_uint32), ("sampler", C.c_uint32), ("geometry", C.c_uint32), ("primitives", C.c_uint32),
     ]
 
 

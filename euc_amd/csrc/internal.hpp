@@ -49,18 +49,38 @@ constexpr uint32_t kRec0Quads = 4, kRec1Quads = 2;
 
 enum FrameFlags : uint32_t { FLAG_BIN_OVERFLOW = 1u };
 
-// Device counters of one launch stream.  ovf[f & 1] counts the overflow
-// entries of frame f; the raster pass of frame f zeroes ovf[(f + 1) & 1] and
-// next[(f + 1) & 1] for the next one.  Each counter sits on its own 128-byte line.
+// Device counters of one launch stream, per frame parity f & 1: setup of frame f
+// counts into the [f & 1] entries, its raster pass reads them and zeroes the
+// [(f + 1) & 1] entries for the next frame.  Each counter sits on its own
+// 128-byte line.
 struct Counters {
-    uint32_t ovf[2][32];
-    uint32_t next[2][8][32];  // per frame parity and XCD: next tile of the dynamic tile loop
-    uint32_t heavy[2][32];    // per frame parity: entries of the heavy-tile list
-    uint32_t big[2][32];      // per frame parity: entries of the big-triangle list
-    uint32_t busy[2][8][32];  // per frame parity and XCD: entries of the busy-tile list
-    uint32_t flags;      // FrameFlags, sticky until the host clears them
-    uint32_t max_count;  // largest per-tile triangle count seen by a one-request frame
-    uint32_t pad[30];
+    uint32_t pool[2][32];     // words of the overflow pool taken
+    uint32_t next[2][8][32];  // per XCD: next tile of the dynamic tile loop
+    uint32_t heavy[2][32];    // entries of the heavy-tile list
+    uint32_t big[2][32];      // entries of the big-triangle list
+    uint32_t busy[2][8][32];  // per XCD: entries of the busy-tile list
+    uint32_t flags[2][32];    // FrameFlags of the frame (the pool ran out: it was dropped)
+    uint32_t max_count[2][32];// largest bin count of a tile past its capacity (one-request frames)
+};
+
+// What a one-request frame's raster pass reports to host-visible memory when
+// it starts (the counts of its own setup): the host grows bins and the pool
+// from it without waiting for the stream.  `frame` is written last.
+struct Report {
+    uint32_t max_count;  // largest tile count past its bin capacity
+    uint32_t flags;      // FrameFlags
+    uint32_t pool_used;  // pool words the frame asked for (more than pool_cap when it ran out)
+    uint32_t dropped;    // frames dropped so far on this context (running count)
+    uint32_t frame;      // the frame these counts belong to
+};
+
+// Per-tile bin state, 8 bytes per tile so the raster pass loads both at once:
+// `count` bin entries (ranks below the tile's capacity sit in its bin), and the
+// head of the tile's overflow segments (pool word + 1, 0 = none).  Every setup
+// workgroup whose ranks for the tile pass the capacity takes one segment of
+// the pool, [next head, n, n primitives], and pushes it on the tile's list.
+struct TileBin {
+    uint32_t count, head;
 };
 
 // Bin entries the raster pass stages per step.
@@ -117,21 +137,35 @@ struct View {
     float u[kUniforms];
 };
 
+// Sampler kinds (EUCHIP_SAMPLER_*): LINEAR + wrap, NEAREST + wrap; wrap 0 none,
+// 1 Clamped, 2 Tiled, 3 Mirrored.
+enum SamplerKind : uint32_t { SAMPLER_LINEAR = 1, SAMPLER_LINEAR_CLAMPED = 2, SAMPLER_NEAREST = 5 };
+enum Geometry : uint32_t { GEOM_IDENTITY = 0, GEOM_REVERSED = 1, GEOM_BOTH_SIDES = 2 };
+
 struct FrameParams {
     // pipeline state shared by every request of the launch
     uint32_t shader, xform, pixel_write, depth_test, depth_write, y_up, z_clip;
     float z_min, z_max;
     uint32_t cull, msaa_level, band_par;
+    uint32_t bgra;         // blend packs [b, g, r, a]
+    uint32_t sampler;      // texture sampler kind (resolved: never EUCHIP_SAMPLER_DEFAULT)
+    uint32_t geometry;     // Geometry: primitive t of the stream is stage output t % k of input triangle t / k
+    // lines (one-request frames): LINE_LIST or LINE_TRIANGLE_LIST, n_lines lines;
+    // a 64-byte record per line and a 64-bit winner key per target pixel
+    uint32_t primitives, n_lines;
+    uint32_t *lrec;
+    uint64_t *lkey;
     // per-launch scratch (device)
     uint4 *rec0;           // kRec0Quads per triangle of the launch
     uint4 *rec1;           // kRec1Quads per triangle of the launch
     float *vary;           // VertexData per shaded vertex
     Counters *ctr;
-    uint32_t *tile_count;  // per tile of the launch, zero before setup, read-only in raster
+    TileBin *tile_count;   // per tile of the launch, zero before setup, read-only in raster
     uint32_t *bins;        // bin_base + local tile * tile_cap + rank -> primitive
-    uint32_t *req_max;     // per request: largest tile count above its tile_cap (host grows the bins)
-    uint2 *ovf;            // (tile, primitive) of ranks >= tile_cap
-    uint32_t ovf_cap;
+    uint32_t *req_max;     // batches, per request: largest tile count above its tile_cap (host grows the bins)
+    uint32_t *pool;        // overflow segments (ranks >= tile_cap), pool_cap words
+    uint32_t pool_cap;
+    Report *report;        // host-visible report of one-request frames, or null
     uint32_t *heavy;       // tiles with at least kHeavyTile entries (heavy_cap slots)
     uint32_t heavy_cap;    // 0: no heavy-first pass
     // Setup lists the tiles it bins into (busy) in 8 lists of up to ntiles
@@ -144,7 +178,7 @@ struct FrameParams {
     uint32_t busy_map;
     uint32_t *big;         // big triangles (primitive ids), big_cap slots; big_cap 0: off (batches)
     uint32_t big_cap;
-    uint32_t *count_next;
+    TileBin *count_next;
     uint32_t zero_n;
     uint32_t frame;        // frame parity selects the overflow counter and tile queue
     uint32_t ntiles;       // tiles of all requests
@@ -163,5 +197,14 @@ struct FrameParams {
 hipError_t launch_setup(const FrameParams &p, hipStream_t s, bool batched, uint32_t blocks);
 hipError_t launch_raster(const FrameParams &p, hipStream_t s, bool batched, uint32_t max_groups);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s);
+// The lines path of one request (p.v0): vertex stage + line setup, winner keys
+// initialised, lines walked, then every pixel resolved (colour, depth, clears).
+hipError_t launch_lines(const FrameParams &p, hipStream_t s);
+
+// PrimitiveKind (EUCHIP_PRIMS_*).
+enum Prims : uint32_t { PRIMS_TRIANGLES = 0, PRIMS_LINE_LIST = 1, PRIMS_LINE_TRIANGLES = 2 };
+// Line endpoints further than this from the origin (pixels) are not drawn.
+constexpr int32_t kLineCoordLimit = 1 << 29;
+constexpr uint32_t kLineRecWords = 16;
 
 }  // namespace euchip

@@ -34,14 +34,17 @@ EUCHIP_DEV bool passes_z_clip(const FrameParams &p, float z) {
     return !p.z_clip || (p.z_min <= z && z <= p.z_max);
 }
 
-// Inclusive wave64 prefix sum.
+// Inclusive wave64 prefix sum on DPP lane moves: shifts of 1, 2, 4 and 8 lanes
+// inside each 16-lane row (lanes without a source add the `old` operand, 0),
+// then row_bcast:15 adds row 0's (row 2's) total to row 1 (row 3) and
+// row_bcast:31 adds lane 31's running total to rows 2 and 3.
 EUCHIP_DEV uint32_t wave_inclusive_sum(uint32_t v) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t o = __shfl_up(v, d, 64);
-        if (lane >= (uint32_t)d) v += o;
-    }
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15 -> rows 1, 3
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31 -> rows 2, 3
     return v;
 }
 
@@ -80,6 +83,17 @@ EUCHIP_DEV void for_each_tile(bool act, uint32_t tx0, uint32_t ty0, uint32_t nx,
     }
 }
 
+// Stream element of corner i of primitive t after the fixed geometry stage
+// (Pipeline::geometry, pipeline.rs:221-229): output t % k of input triangle
+// t / k, [a, b, c] or [a, c, b].
+EUCHIP_DEV uint32_t corner_element(uint32_t geometry, uint32_t t, int i) {
+    uint32_t tri = t, rev = 0u;
+    if (geometry == GEOM_BOTH_SIDES) tri = t >> 1, rev = t & 1u;
+    else if (geometry == GEOM_REVERSED) rev = 1u;
+    const uint32_t c = (i == 0 || !rev) ? (uint32_t)i : 3u - (uint32_t)i;
+    return 3u * tri + c;
+}
+
 // ---------------------------------------------------------------------------
 // triangle setup (triangles.rs:55-171) of primitive t of request v, whole-target
 // band clamp, from the vertex shader's clip positions.  Returns false when the
@@ -94,7 +108,7 @@ EUCHIP_DEV bool tri_setup(const FrameParams &p, const View &v, uint32_t t, TriRe
     // loads of all three corners are in flight together
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const uint32_t e = 3u * t + (uint32_t)i;
+        const uint32_t e = corner_element(p.geometry, t, i);
         inst[i] = e / v.count;
         vi[i] = e - inst[i] * v.count;
     }
@@ -234,16 +248,26 @@ EUCHIP_DEV bool tri_setup(const FrameParams &p, const View &v, uint32_t t, TriRe
     return R.xmin < R.xmax && R.ymin < R.ymax;
 }
 
-// Where rank `rank` of tile `tile` (launch tile space) of request v goes: its
-// bin slot, or the overflow list past the tile's capacity.
-EUCHIP_DEV void bin_store(const FrameParams &p, const View &v, uint32_t tile, uint32_t rank, uint32_t prim) {
-    if (rank < v.tile_cap) {
-        p.bins[v.bin_base + (size_t)(tile - v.tile_base) * v.tile_cap + rank] = prim;
-    } else {
-        const uint32_t o = atomicAdd(&p.ctr->ovf[p.frame & 1u][0], 1u);
-        if (o < p.ovf_cap) p.ovf[o] = make_uint2(tile, prim);
-        else atomicOr(&p.ctr->flags, FLAG_BIN_OVERFLOW);
+// A segment of n overflow entries of `tile`, taken from the pool and pushed on
+// the tile's list: returns the pool index of its first entry, or ~0u when the
+// pool ran out (the frame is then dropped and reported; the host re-runs a
+// blocking render with a larger pool).
+EUCHIP_DEV uint32_t take_segment(const FrameParams &p, uint32_t tile, uint32_t n) {
+    const uint32_t words = n + 2u;
+    const uint32_t base = atomicAdd(&p.ctr->pool[p.frame & 1u][0], words);
+    if (base >= p.pool_cap || p.pool_cap - base < words) {
+        atomicOr(&p.ctr->flags[p.frame & 1u][0], FLAG_BIN_OVERFLOW);
+        return ~0u;
     }
+    p.pool[base] = atomicExch(&p.tile_count[tile].head, base + 1u);
+    p.pool[base + 1u] = n;
+    return base + 2u;
+}
+
+// Bin slot of rank `rank` (below the capacity) of tile `tile` (launch tile space)
+// of request v.
+EUCHIP_DEV uint32_t *bin_slot(const FrameParams &p, const View &v, uint32_t tile, uint32_t rank) {
+    return p.bins + v.bin_base + (size_t)(tile - v.tile_base) * v.tile_cap + rank;
 }
 
 // Range [beg, end) of this XCD's tiles: the hardware deals workgroups
@@ -288,9 +312,10 @@ EUCHIP_DEV void note_heavy(const FrameParams &p, uint32_t tile, uint32_t old, ui
     }
 }
 
-// Bin append of one pair by a global atomic (the fallback of the LDS-grouped path).
-EUCHIP_DEV void bin_direct(const FrameParams &p, const View &v, uint32_t tile, uint32_t prim) {
-    const uint32_t rank = atomicAdd(p.tile_count + tile, 1u);
+// Bin append of one pair by a global atomic (the fallback of the LDS-grouped
+// path).  maxp: where counts past the capacity are recorded for the host.
+EUCHIP_DEV void bin_direct(const FrameParams &p, const View &v, uint32_t tile, uint32_t prim, uint32_t *maxp) {
+    const uint32_t rank = atomicAdd(&p.tile_count[tile].count, 1u);
     note_heavy(p, tile, rank, 1u);
     if (p.busy && rank == 0u) {
         const uint32_t x = busy_list_of(p, tile);
@@ -299,7 +324,13 @@ EUCHIP_DEV void bin_direct(const FrameParams &p, const View &v, uint32_t tile, u
         const uint32_t k = atomicAdd(&p.ctr->busy[p.frame & 1u][x][0], 1u);
         if (k < lc) p.busy[(size_t)lb + k] = tile;
     }
-    bin_store(p, v, tile, rank, prim);
+    if (rank < v.tile_cap) {
+        *bin_slot(p, v, tile, rank) = prim;
+    } else {
+        atomicMax(maxp, rank + 1u);
+        const uint32_t e = take_segment(p, tile, 1u);
+        if (e != ~0u) p.pool[e] = prim;
+    }
 }
 
 // The request a workgroup or tile works on: the kernel argument's, or one of
@@ -352,10 +383,10 @@ EUCHIP_DEV void load_rec1(const FrameParams &p, uint32_t rid, TriRec &R) {
 }
 
 // Shaded-vertex ids of primitive t's corners after the winding swap.
-EUCHIP_DEV void tri_sids(const View &v, uint32_t t, uint32_t flags, uint32_t sid[3]) {
+EUCHIP_DEV void tri_sids(const View &v, uint32_t geometry, uint32_t t, uint32_t flags, uint32_t sid[3]) {
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const uint32_t e = 3u * t + (uint32_t)i;
+        const uint32_t e = corner_element(geometry, t, i);
         const uint32_t inst = e / v.count;
         const uint32_t k = e - inst * v.count;
         sid[i] = v.sv_base + inst * v.n_vertices + (v.idx ? v.idx[k] : k);
@@ -408,6 +439,7 @@ EUCHIP_DEV bool covers_any(const FrameParams &p, const TriRec &R) {
 
 constexpr uint32_t kHash = 512;    // LDS hash slots (distinct tiles) per setup workgroup
 constexpr uint32_t kPairs = 1024;  // (tile, primitive) pairs grouped in LDS per workgroup
+static_assert(kHash == 512u && kPairs <= (1u << 23), "pairs pack a 9-bit hash slot and their rank in the slot");
 
 // ---------------------------------------------------------------------------
 // k_setup: per-vertex clip position + VertexData (thread t = shaded vertex t)
@@ -424,7 +456,8 @@ union SetupShared {
     } rec;
     struct {
         uint32_t hkey[kHash], hcnt[kHash], hbase[kHash];
-        uint32_t pl_prim[kPairs], pl_slot[kPairs], pl_local[kPairs];
+        uint32_t hseg[kHash];                  // pool index of the slot's overflow entries
+        uint32_t pl_prim[kPairs], pl_sl[kPairs];  // pair: primitive; hash slot | rank in the slot << 9
     } bin;
 };
 
@@ -462,10 +495,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
             pack_rec(R, q0, q1);
 #pragma unroll
             for (uint32_t k = 0; k < kRec0Quads; k++) sm.rec.q0[tid * kRec0Quads + k] = q0[k];
-            // rec1: this lane's 32 bytes, contiguous with its neighbours' (no staging)
+            // rec1: this lane's 32 bytes, contiguous with its neighbours' (no
+            // staging); only triangles with a span row read it (all_rows_boxed)
+            if (!all_rows_boxed(R)) {
 #pragma unroll
-            for (uint32_t k = 0; k < kRec1Quads; k++)
-                store_stream(&p.rec1[((size_t)v.tri_base + t) * kRec1Quads + k], q1[k]);
+                for (uint32_t k = 0; k < kRec1Quads; k++)
+                    store_stream(&p.rec1[((size_t)v.tri_base + t) * kRec1Quads + k], q1[k]);
+            }
         }
     }
     sm.rec.kept[tid] = keep ? 1u : 0u;
@@ -498,6 +534,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     if (tid == 0u) npl = 0u;
     __syncthreads();
     const uint32_t tbase = v.tile_base;
+    uint32_t *maxp = BATCH ? p.req_max + r : &p.ctr->max_count[p.frame & 1u][0];
     for_each_tile(keep, tx0, ty0, nx, ny, v.ntx, t, 0u, [&](uint32_t lt, uint32_t, uint32_t prim, uint32_t) {
         const uint32_t tile = tbase + lt;
         uint32_t h = (tile * 2654435761u) >> 23;  // 9-bit multiplicative hash
@@ -507,15 +544,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                 const uint32_t k = atomicAdd(&npl, 1u);
                 if (k < kPairs) {
                     sm.bin.pl_prim[k] = prim;
-                    sm.bin.pl_slot[k] = h;
-                    sm.bin.pl_local[k] = atomicAdd(&sm.bin.hcnt[h], 1u);
+                    sm.bin.pl_sl[k] = h | (atomicAdd(&sm.bin.hcnt[h], 1u) << 9);
                     return;
                 }
                 break;
             }
             h = (h + 1u) & (kHash - 1u);
         }
-        bin_direct(p, v, tile, prim);  // table or pair list full
+        bin_direct(p, v, tile, prim, maxp);  // table or pair list full
     });
     __syncthreads();
     // One global atomic per distinct tile; tiles that became busy or heavy get a
@@ -527,8 +563,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         const uint32_t n = sm.bin.hcnt[i];
         if (n) {
             const uint32_t tile = sm.bin.hkey[i] - 1u;
-            const uint32_t old = atomicAdd(p.tile_count + tile, n);
+            const uint32_t old = atomicAdd(&p.tile_count[tile].count, n);
             sm.bin.hbase[i] = old;
+            // ranks past the capacity: one overflow segment for this workgroup's share
+            if (old + n > v.tile_cap) {
+                atomicMax(maxp, old + n);
+                const uint32_t first = old > v.tile_cap ? old : v.tile_cap;
+                sm.bin.hseg[i] = take_segment(p, tile, old + n - first);
+            }
             uint32_t code = 0u;
             if (p.busy && old == 0u) {
                 const uint32_t x = busy_list_of(p, tile);
@@ -550,8 +592,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
     const uint32_t np = npl < kPairs ? npl : kPairs;
     for (uint32_t k = tid; k < np; k += 256u) {
-        const uint32_t h = sm.bin.pl_slot[k];
-        bin_store(p, v, sm.bin.hkey[h] - 1u, sm.bin.hbase[h] + sm.bin.pl_local[k], sm.bin.pl_prim[k]);
+        const uint32_t h = sm.bin.pl_sl[k] & (kHash - 1u), rank = sm.bin.hbase[h] + (sm.bin.pl_sl[k] >> 9);
+        if (rank < v.tile_cap) {
+            *bin_slot(p, v, sm.bin.hkey[h] - 1u, rank) = sm.bin.pl_prim[k];
+        } else {
+            const uint32_t base = sm.bin.hbase[h], first = base > v.tile_cap ? base : v.tile_cap;
+            const uint32_t e = sm.bin.hseg[h];
+            if (e != ~0u) p.pool[e + (rank - first)] = sm.bin.pl_prim[k];
+        }
     }
     if (!any_new) return;  // no tile became busy or heavy here (uniform)
     if (tid < 9u) lbase[tid] = my_base;
@@ -692,7 +740,7 @@ EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const View &v, const TriRe
     const uint32_t L = p.msaa_level;
     if (L == 0u) {
         v_data<SH>(R, V, (float)px, (float)py, vd);
-        fragment_shader<SH>(v, vd, frag);
+        fragment_shader<SH>(v, p.sampler, vd, frag);
     } else {
         // coarse shading on the band-relative 2^L grid with bilinear reconstruction
         // (pipeline.rs:541-570); tgt_min = (0, by0)
@@ -706,7 +754,7 @@ EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const View &v, const TriRe
         for (int q = 0; q < 4; q++) {
             const uint32_t ix = cx + (uint32_t)(q & 1), iy = cy + (uint32_t)(q >> 1);
             v_data<SH>(R, V, (float)(ix << L), (float)(by0 + (iy << L)), vd);
-            fragment_shader<SH>(v, vd, t[q]);
+            fragment_shader<SH>(v, p.sampler, vd, t[q]);
         }
         if constexpr (SH == FLAT) {
             frag[0] = t[0][0];
@@ -719,7 +767,7 @@ EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const View &v, const TriRe
             }
         }
     }
-    return blend_shader<SH>(old, frag);
+    return blend_shader<SH>(old, frag, p.bgra);
 }
 
 // Tile geometry: TW x TH pixels at (X0, Y0); TW = 1 << sx, TH = 256 >> sx.
@@ -897,35 +945,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     const uint32_t par = p.frame & 1u, npar = (p.frame + 1u) & 1u;
     const uint32_t xcd = blockIdx.x & 7u;
     const uint32_t per_xcd = gridDim.x >> 3, w = blockIdx.x >> 3;
-    // The loads the first tile waits for, issued together: frame flags, overflow
-    // count and (one-request frames) list lengths, this workgroup's first busy
-    // tile and its count (read at a clamped index before the tile is known to
-    // be in the list; used only if it is).
+    // The loads the first tile waits for, issued together: frame flags, pool use,
+    // list lengths, this workgroup's first busy tile and its bin state (read at a
+    // clamped index before the tile is known to be in the list; used only if it is).
     uint32_t lbase, lcap;
     busy_list_span(p, xcd, lbase, lcap);
     const uint32_t *list = p.busy + lbase;  // this XCD's busy list
-    const uint32_t flags = p.ctr->flags;
-    const uint32_t novf0 = p.ctr->ovf[par][0];
+    const uint32_t flags = p.ctr->flags[par][0];
+    const uint32_t pool_used0 = p.ctr->pool[par][0];
     uint32_t nh = __builtin_amdgcn_readfirstlane(p.ctr->heavy[par][0]);
     uint32_t nbig = BIG ? __builtin_amdgcn_readfirstlane(p.ctr->big[par][0]) : 0u;
     uint32_t nbusy = __builtin_amdgcn_readfirstlane(p.ctr->busy[par][xcd][0]);
     const uint32_t first = w < lcap ? __builtin_amdgcn_readfirstlane(list[w]) : 0u;
-    const uint32_t first_cnt = __builtin_amdgcn_readfirstlane(p.tile_count[first < ntiles ? first : 0u]);
+    const TileBin first_bin = p.tile_count[first < ntiles ? first : 0u];
+    if (p.report && blockIdx.x == 0u && tid == 0u) {
+        // this frame's setup counts, for the host (frame number last)
+        Report *rp = p.report;
+        rp->max_count = p.ctr->max_count[par][0];
+        rp->flags = flags;
+        rp->pool_used = pool_used0;
+        if (flags & FLAG_BIN_OVERFLOW) rp->dropped = rp->dropped + 1u;
+        __threadfence_system();
+        rp->frame = p.frame;
+    }
+    // The next frame's counters start at zero (its setup counts into them).
+    if (blockIdx.x == 0u && tid == 0u) {
+        p.ctr->pool[npar][0] = 0u;
+        p.ctr->flags[npar][0] = 0u;
+        p.ctr->max_count[npar][0] = 0u;
+        p.ctr->heavy[npar][0] = 0u;
+        p.ctr->big[npar][0] = 0u;
+    }
+    if (blockIdx.x < 8u && tid == 0u) {
+        p.ctr->next[npar][xcd][0] = 0u;
+        p.ctr->busy[npar][xcd][0] = 0u;
+    }
+    for (uint32_t i = blockIdx.x * 256u + tid; i < p.zero_n; i += gridDim.x * 256u) p.count_next[i] = TileBin{0u, 0u};
+    // The pool ran out in setup: entries were lost, the frame is dropped (the
+    // host sees the flag and re-runs it with a larger pool).
     if (flags & FLAG_BIN_OVERFLOW) return;
-    if (blockIdx.x == 0u && tid == 0u) p.ctr->ovf[npar][0] = 0u;  // next frame's list
-    const uint32_t novf = novf0 < p.ovf_cap ? novf0 : p.ovf_cap;
+    const uint32_t pool_used = pool_used0 < p.pool_cap ? pool_used0 : p.pool_cap;
     const bool uses_depth = p.depth_test != DT_NONE || p.depth_write;
     const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
                         : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
     const uint32_t sx = p.tile_sx, sy = 8u - sx;
     uint32_t *next = &p.ctr->next[par][xcd][0];
-    if (blockIdx.x < 8u && tid == 0u) p.ctr->next[npar][xcd][0] = 0u;  // next frame's queue
 
-    // One tile with `cnt` bin entries: winner keys in LDS, then the resolve.
+    // One tile with `cnt` bin entries and overflow list `head`: winner keys in
+    // LDS, then the resolve.
     // With `claim`, thread 0 takes the next queue index when the keys are done
     // (the atomic's latency hides behind the resolve) and leaves it in sh.next.
     // Ends with a barrier (LDS is reused by the next tile).
-    auto run_tile = [&](uint32_t tile, uint32_t cnt, bool claim) {
+    auto run_tile = [&](uint32_t tile, uint32_t cnt, uint32_t head, bool claim) {
         uint32_t r = 0;
         if constexpr (BATCH) r = __builtin_amdgcn_readfirstlane(p.tile_req[tile]);
         const View &v = view_of<BATCH>(p, r);
@@ -949,13 +1020,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         }
         const uint32_t cap = v.tile_cap;
         const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
-        if (tid == 0u && cnt > cap) atomicMax(p.req_max + r, cnt);
         __syncthreads();
-        // this tile's bin, then (only when it overflowed) its entries of the
-        // overflow list, then the big triangles (staging drops those that miss it)
+        // this tile's bin, then the big triangles (staging drops those that miss
+        // it), then (only when it overflowed) its overflow segments
         const uint32_t nb = cnt < cap ? cnt : cap;
-        const uint32_t no = cnt > cap ? novf : 0u;
-        const uint32_t nscan = nb + no + nbig;
+        const uint32_t nscan = nb + nbig;
         for (uint32_t b = 0; b < nscan; b += kBatch) {
             const uint32_t i = b + tid;
             bool valid = false;
@@ -963,13 +1032,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             if (tid >= kBatch || i >= nscan) {
             } else if (i < nb) {
                 valid = true, prim = bin[i];
-            } else if (i < nb + no) {
-                const uint2 e = p.ovf[i - nb];
-                valid = e.x == tile, prim = e.y;
             } else {
-                valid = BIG, prim = BIG ? p.big[i - nb - no] : 0u;
+                valid = BIG, prim = BIG ? p.big[i - nb] : 0u;
             }
             raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, prim, g, mode);
+        }
+        // Segments were taken from the pool in this frame, each pointing at the
+        // one taken before it; a walk of at most pool_used / 3 steps whose every
+        // read stays below pool_used ends even on a damaged list.
+        uint32_t seg = cnt > cap ? head : 0u;
+        for (uint32_t guard = 0; seg && guard <= pool_used / 3u; guard++) {
+            const uint32_t s0 = seg - 1u;
+            if (s0 + 2u > pool_used) break;
+            const uint32_t nxt = __builtin_amdgcn_readfirstlane(p.pool[s0]);
+            uint32_t len = __builtin_amdgcn_readfirstlane(p.pool[s0 + 1u]);
+            len = len < pool_used - (s0 + 2u) ? len : pool_used - (s0 + 2u);
+            for (uint32_t b = 0; b < len; b += kBatch) {
+                const uint32_t i = b + tid;
+                const bool valid = tid < kBatch && i < len;
+                raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, valid ? p.pool[s0 + 2u + i] : 0u, g, mode);
+            }
+            seg = nxt;
         }
         uint32_t claimed = 0u;
         if (claim && tid == 0u) claimed = per_xcd + atomicAdd(next, 1u);
@@ -992,7 +1075,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
                 if (p.pixel_write && has && SH != DEPTH_ONLY) {
                     TriRec R;
                     load_rec0(p, v.tri_base + prim, R);
-                    tri_sids(v, prim, R.flags, R.sid);
+                    tri_sids(v, p.geometry, prim, R.flags, R.sid);
                     store_stream(&v.pixel[pidx], shade_pixel<SH>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
                                                     v.clear_color_en ? v.clear_color : 0u));
                 } else if (v.clear_color_en) {
@@ -1008,12 +1091,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         __syncthreads();
     };
 
-    // The next frame's counts start at zero (stores only, issued first).
-    for (uint32_t i = blockIdx.x * 256u + tid; i < p.zero_n; i += gridDim.x * 256u) p.count_next[i] = 0u;
-    // next frame's lists start empty (every instantiation: the next frame may be
-    // in big-triangle mode when this one is not)
-    if (blockIdx.x == 0u && tid == 0u) p.ctr->heavy[npar][0] = 0u, p.ctr->big[npar][0] = 0u;
-    if (blockIdx.x < 8u && tid == 0u) p.ctr->busy[npar][xcd][0] = 0u;
     if constexpr (BIG) nbig = nbig < p.big_cap ? nbig : p.big_cap;
     // With big triangles every tile may hold fragments: the loop below then takes
     // every tile of this XCD's contiguous range instead of its busy list.
@@ -1028,7 +1105,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     //    start early instead of ending the launch late).
     for (uint32_t k = blockIdx.x; k < nh; k += gridDim.x) {
         const uint32_t tile = __builtin_amdgcn_readfirstlane(p.heavy[k]);
-        run_tile(tile, __builtin_amdgcn_readfirstlane(p.tile_count[tile]), false);
+        const TileBin tb = p.tile_count[tile];
+        run_tile(tile, __builtin_amdgcn_readfirstlane(tb.count), __builtin_amdgcn_readfirstlane(tb.head), false);
     }
     // 2. Busy tiles of this XCD's list: the first by workgroup index, the rest
     //    from the XCD's counter (heavy ones are done).
@@ -1041,9 +1119,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         } else {
             tile = k == w ? first : __builtin_amdgcn_readfirstlane(list[k]);
         }
-        const uint32_t cnt = k == w && !nbig ? first_cnt : __builtin_amdgcn_readfirstlane(p.tile_count[tile]);
+        const TileBin tb = k == w && !nbig ? first_bin : p.tile_count[tile];
+        const uint32_t cnt = __builtin_amdgcn_readfirstlane(tb.count);
         if (!(p.heavy_cap && cnt >= kHeavyTile)) {
-            run_tile(tile, cnt, true);
+            run_tile(tile, cnt, __builtin_amdgcn_readfirstlane(tb.head), true);
         } else {
             if (tid == 0u) sh.next = per_xcd + atomicAdd(next, 1u);
             __syncthreads();
@@ -1061,7 +1140,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
 #pragma unroll
         for (uint32_t j = 0; j < 8u; j++) {
             const uint32_t t = base + j * per_xcd;
-            c[j] = t < t1 ? p.tile_count[t] : 1u;
+            c[j] = t < t1 ? p.tile_count[t].count : 1u;
         }
 #pragma unroll
         for (uint32_t j = 0; j < 8u; j++) {
@@ -1165,6 +1244,291 @@ hipError_t launch_raster(const FrameParams &p, hipStream_t s, bool batched, uint
     case TEXTURED: launch_raster_t<TEXTURED>(p, s, batched, g, cus); break;
     case PHONG_SHADOW: launch_raster_t<PHONG_SHADOW>(p, s, batched, g, cus); break;
     case TEX_SHADOW: launch_raster_t<TEX_SHADOW>(p, s, batched, g, cus); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------------------
+// lines (lines.rs:15-121): one-request frames of LineList / LineTriangleList
+// ---------------------------------------------------------------------------
+
+// Clip position and shaded-vertex id of stream element e of request v.
+EUCHIP_DEV void element_clip(const FrameParams &p, const View &v, uint32_t e, float c[4], uint32_t &sid) {
+    const uint32_t inst = e / v.count;
+    uint32_t vi = e - inst * v.count;
+    if (v.idx) vi = v.idx[vi];
+    const float *P = v.pos + (size_t)v.pos_comps * vi;
+    if (p.xform == 0u) {
+        c[0] = P[0], c[1] = P[1], c[2] = P[2], c[3] = v.pos_comps == 4u ? P[3] : 1.0f;
+    } else {
+        const float p4[4] = {P[0], P[1], P[2], 1.0f};
+        float w[4];
+        mat4_vec4(v.models + 16u * inst, p4, w);
+        mat4_vec4(v.u + U_VP, w, c);
+    }
+    sid = v.sv_base + inst * v.n_vertices + vi;
+}
+
+// `f as isize` clamped to what the record keeps: NaN -> 0, otherwise truncation,
+// saturated (coordinates past kLineCoordLimit mark the line undrawn).
+EUCHIP_DEV int32_t line_coord(float f, bool &far) {
+    if (f != f) return 0;
+    if (!(f > -(float)kLineCoordLimit && f < (float)kLineCoordLimit)) {
+        far = true;
+        return 0;
+    }
+    return (int32_t)f;
+}
+
+// f32::clamp then `as isize` for a bound within [lo, hi] (lo, hi integral, >= 0).
+EUCHIP_DEV int32_t clamp_coord(float f, float lo, float hi) {
+    if (f < lo) f = lo;
+    if (f > hi) f = hi;
+    if (f != f) return 0;
+    return (int32_t)f;
+}
+
+// k_line_setup: the vertex stage (lane t = shaded vertex t) and one line record
+// per lane (lane t = line t): lines.rs:41-86 in f32 over the whole target (the
+// band clamp of the clip rectangle only cuts rows other bands draw).
+template <uint32_t SH>
+__global__ __launch_bounds__(256) void k_line_setup(FrameParams p) {
+    const View &v = p.v0;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if constexpr (vd_count(SH) > 0) {
+        if (t < v.n_instances * v.n_vertices) {
+            const uint32_t inst = t / v.n_vertices;
+            float vd[VDN<SH>::N];
+            vertex_data<SH>(p, v, inst, t - inst * v.n_vertices, vd);
+#pragma unroll
+            for (uint32_t q = 0; q < vd_count(SH); q++) p.vary[((size_t)v.sv_base + t) * vd_count(SH) + q] = vd[q];
+        }
+    }
+    if (t >= p.n_lines) return;
+    uint32_t e0, e1;
+    if (p.primitives == PRIMS_LINE_LIST) {
+        e0 = 2u * t, e1 = 2u * t + 1u;
+    } else {  // edges a-b, b-c, c-a of stage-output triangle t / 3 (primitives.rs:71-84)
+        const uint32_t tri = t / 3u, j = t - 3u * tri;
+        e0 = corner_element(p.geometry, tri, (int)j);
+        e1 = corner_element(p.geometry, tri, j == 2u ? 0 : (int)j + 1);
+    }
+    float hom[2][4];
+    uint32_t sid[2];
+    element_clip(p, v, e0, hom[0], sid[0]);
+    element_clip(p, v, e1, hom[1], sid[1]);
+    const float flip1 = p.y_up ? -1.0f : 1.0f;
+    const float size0 = (float)v.W, size1 = (float)v.H;
+    float ez[2], vs[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const float x = hom[i][0] * 1.0f, y = hom[i][1] * flip1, z = hom[i][2];
+        const float w = rmax(hom[i][3], 0.0001f);
+        const float ex = x / w, ey = y / w;
+        ez[i] = z / w;
+        vs[i][0] = size0 * (ex * 0.5f + 0.5f);
+        vs[i][1] = size1 * (ey * -0.5f + 0.5f);
+    }
+    bool far = false;
+    const int32_t x1 = line_coord(vs[0][0], far), y1 = line_coord(vs[0][1], far);
+    const int32_t x2 = line_coord(vs[1][0], far), y2 = line_coord(vs[1][1], far);
+    const int32_t wx1 = clamp_coord(rmin(vs[0][0], vs[1][0]) + 0.0f, 0.0f, size0);
+    const int32_t wy1 = clamp_coord(rmin(vs[0][1], vs[1][1]) + 0.0f, 0.0f, size1);
+    const int32_t wx2 = clamp_coord(rmax(vs[0][0], vs[1][0]) + 1.0f, 0.0f, size0);
+    const int32_t wy2 = clamp_coord(rmax(vs[0][1], vs[1][1]) + 1.0f, 0.0f, size1);
+    const int32_t adx = x1 > x2 ? x1 - x2 : x2 - x1, ady = y1 > y2 ? y1 - y2 : y2 - y1;
+    const bool use_x = adx > ady;
+    const float norm = 1.0f / (use_x ? vs[1][0] - vs[0][0] : vs[1][1] - vs[0][1]);
+    uint32_t *r = p.lrec + (size_t)t * kLineRecWords;
+    const uint4 q0 = make_uint4(__float_as_uint(vs[0][0]), __float_as_uint(vs[0][1]), __float_as_uint(norm),
+                                __float_as_uint(ez[0]));
+    const uint4 q1 = make_uint4(__float_as_uint(ez[1]), (uint32_t)x1, (uint32_t)y1, (uint32_t)x2);
+    const uint4 q2 = make_uint4((uint32_t)y2, (uint32_t)wx1, (uint32_t)(wx2 - 1), (uint32_t)wy1);
+    const uint4 q3 = make_uint4((uint32_t)(wy2 - 1), (use_x ? 1u : 0u) | (far ? 0u : 2u), sid[0], sid[1]);
+    uint4 *rq = reinterpret_cast<uint4 *>(r);
+    rq[0] = q0, rq[1] = q1, rq[2] = q2, rq[3] = q3;
+}
+
+// k_line_init: every pixel's winner key from the depth before the pass.
+__global__ __launch_bounds__(256) void k_line_init(FrameParams p) {
+    const View &v = p.v0;
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= (size_t)v.W * v.H) return;
+    const bool uses_depth = p.depth_test != DT_NONE || p.depth_write;
+    const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
+                        : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
+    float D0 = 0.0f;
+    if (uses_depth) D0 = v.clear_depth_en ? v.clear_depth : v.depth[i];
+    p.lkey[i] = init_key(mode, D0);
+}
+
+// k_line_raster: one wave per line, its lanes on the pixels k, k + 64, ... of
+// the clipped Bresenham walk (clipline 0.2's contract: the unclipped line's
+// pixels inside the rectangle), each updating its pixel's winner key.
+__global__ __launch_bounds__(256) void k_line_raster(FrameParams p) {
+    const View &v = p.v0;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6), nwaves = gridDim.x * 4u;
+    const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
+                        : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
+    for (uint32_t l = wave; l < p.n_lines; l += nwaves) {
+        const uint4 *rq = reinterpret_cast<const uint4 *>(p.lrec + (size_t)l * kLineRecWords);
+        const uint4 q0 = rq[0], q1 = rq[1], q2 = rq[2], q3 = rq[3];
+        if (!(q3.y & 2u)) continue;  // an endpoint past the coordinate limit
+        const float vs0x = __uint_as_float(q0.x), vs0y = __uint_as_float(q0.y), norm = __uint_as_float(q0.z);
+        const float e0z = __uint_as_float(q0.w), e1z = __uint_as_float(q1.x);
+        const int64_t x1 = (int32_t)q1.y, y1 = (int32_t)q1.z, x2 = (int32_t)q1.w, y2 = (int32_t)q2.x;
+        const int64_t cx0 = (int32_t)q2.y, cx1 = (int32_t)q2.z, cy0 = (int32_t)q2.w, cy1 = (int32_t)q3.x;
+        const bool use_x = q3.y & 1u;
+        if (cx1 < cx0 || cy1 < cy0) continue;
+        const int64_t dx = x2 - x1, dy = y2 - y1;
+        const int64_t adx = dx < 0 ? -dx : dx, ady = dy < 0 ? -dy : dy;
+        const bool xmajor = adx >= ady;
+        const int64_t M1 = xmajor ? x1 : y1, m1 = xmajor ? y1 : x1;
+        const int64_t dM = xmajor ? adx : ady, dm = xmajor ? ady : adx;
+        const int64_t sM = (xmajor ? dx : dy) < 0 ? -1 : 1, sm = (xmajor ? dy : dx) < 0 ? -1 : 1;
+        const int64_t c0 = xmajor ? cx0 : cy0, c1 = xmajor ? cx1 : cy1;
+        const int64_t n0 = xmajor ? cy0 : cx0, n1 = xmajor ? cy1 : cx1;
+        int64_t klo = sM > 0 ? c0 - M1 : M1 - c1, khi = sM > 0 ? c1 - M1 : M1 - c0;
+        if (klo < 0) klo = 0;
+        if (khi > dM) khi = dM;
+        for (int64_t k = klo + lane; k <= khi; k += 64) {
+            const int64_t j = dM ? (2 * k * dm + dM - 1) / (2 * dM) : 0;
+            const int64_t M = M1 + sM * k, m = m1 + sm * j;
+            if (m < n0 || m > n1) continue;
+            const uint32_t x = (uint32_t)(xmajor ? M : m), y = (uint32_t)(xmajor ? m : M);
+            // lines.rs:88-95
+            const float frac = (use_x ? (float)x - vs0x : (float)y - vs0y) * norm;
+            const float z = e0z + frac * (e1z - e0z);
+            if (!passes_z_clip(p, z)) continue;
+            const size_t pi = (size_t)y * v.W + x;
+            float D0 = 0.0f;
+            if (mode == 0u && p.depth_test != DT_NONE) D0 = v.clear_depth_en ? v.clear_depth : v.depth[pi];
+            update_key(p.lkey + pi, mode, p.depth_test, z, D0, l);
+        }
+    }
+}
+
+// get_v_data of a line (lines.rs:99-112): weighted_sum2(v0, v1, 1 - frac, frac).
+template <uint32_t SH>
+EUCHIP_DEV void v_data_line(const float (*V)[VDN<SH>::N], bool use_x, float vs0x, float vs0y, float norm, float X,
+                            float Y, float *vd) {
+    const float frac = (use_x ? X - vs0x : Y - vs0y) * norm;
+    const float w0 = 1.0f - frac;
+    if constexpr (SH == FLAT) {
+        vd[0] = V[0][0];
+    } else {
+#pragma unroll
+        for (uint32_t c = 0; c < vd_count(SH); c++) vd[c] = V[0][c] * w0 + V[1][c] * frac;
+    }
+}
+
+// k_line_resolve: every pixel of the target: the winning line's fragment (shaded
+// once, with the coarse-shading combine under MSAA), its depth, or the clears.
+template <uint32_t SH>
+__global__ __launch_bounds__(256) void k_line_resolve(FrameParams p) {
+    const View &v = p.v0;
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= (size_t)v.W * v.H) return;
+    const uint32_t px = (uint32_t)(i % v.W), py = (uint32_t)(i / v.W);
+    const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
+                        : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
+    const uint64_t kk = p.lkey[i];
+    bool has;
+    uint32_t prim;
+    float z;
+    if (mode == 0u) {
+        has = kk != 0ull;
+        prim = (uint32_t)(kk >> 32) - 1u;
+        z = __uint_as_float((uint32_t)kk);
+    } else {
+        has = ((uint32_t)kk) != 0u;
+        prim = ((uint32_t)kk >> 1) - 1u;
+        const uint32_t o = (uint32_t)(kk >> 32);
+        z = zunord(mode == 1u ? o : ~o, (uint32_t)kk & 1u);
+    }
+    if (v.pixel) {
+        if (p.pixel_write && has && SH != DEPTH_ONLY) {
+            const uint4 *rq = reinterpret_cast<const uint4 *>(p.lrec + (size_t)prim * kLineRecWords);
+            const uint4 q0 = rq[0], q3 = rq[3];
+            const float vs0x = __uint_as_float(q0.x), vs0y = __uint_as_float(q0.y), norm = __uint_as_float(q0.z);
+            const bool use_x = q3.y & 1u;
+            constexpr uint32_t NV = VDN<SH>::N;
+            float V[2][NV];
+            const uint32_t sid[2] = {q3.z, q3.w};
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+#pragma unroll
+                for (uint32_t c = 0; c < vd_count(SH); c++) V[k][c] = p.vary[(size_t)sid[k] * vd_count(SH) + c];
+            if constexpr (vd_count(SH) == 0) V[0][0] = V[1][0] = 0.0f;
+            float frag[4], vd[NV];
+            const uint32_t L = p.msaa_level;
+            if (L == 0u) {
+                v_data_line<SH>(V, use_x, vs0x, vs0y, norm, (float)px, (float)py, vd);
+                fragment_shader<SH>(v, p.sampler, vd, frag);
+            } else {
+                uint32_t by0, by1;
+                band_of(p, v, py, by0, by1);
+                const float div = 1.0f / (float)(1u << L);
+                const uint32_t rx = px, ry = py - by0;
+                const float fx = rfract((float)rx * div);
+                const float fy = rfract((float)ry * div);
+                const uint32_t cx = rx >> L, cy = ry >> L;
+                float t[4][4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t ix = cx + (uint32_t)(q & 1), iy = cy + (uint32_t)(q >> 1);
+                    v_data_line<SH>(V, use_x, vs0x, vs0y, norm, (float)(ix << L), (float)(by0 + (iy << L)), vd);
+                    fragment_shader<SH>(v, p.sampler, vd, t[q]);
+                }
+                if constexpr (SH == FLAT) {
+                    frag[0] = t[0][0];
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const float t0 = t[0][c] * (1.0f - fy) + t[2][c] * fy;
+                        const float t1 = t[1][c] * (1.0f - fy) + t[3][c] * fy;
+                        frag[c] = t0 * (1.0f - fx) + t1 * fx;
+                    }
+                }
+            }
+            v.pixel[i] = blend_shader<SH>(v.clear_color_en ? v.clear_color : v.pixel[i], frag, p.bgra);
+        } else if (v.clear_color_en) {
+            v.pixel[i] = v.clear_color;
+        }
+    }
+    if (v.depth) {
+        if (has && p.depth_write) v.depth[i] = z;
+        else if (v.clear_depth_en) v.depth[i] = v.clear_depth;
+    }
+}
+
+template <uint32_t SH>
+static void launch_lines_t(const FrameParams &p, hipStream_t s) {
+    const size_t nsv = (size_t)p.v0.n_instances * p.v0.n_vertices;
+    const size_t npx = (size_t)p.v0.W * p.v0.H;
+    hipLaunchKernelGGL((k_line_setup<SH>), dim3(blocks_of(std::max<size_t>(nsv, p.n_lines), 256)), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_line_init, dim3(blocks_of(npx, 256)), dim3(256), 0, s, p);
+    if (p.n_lines) {
+        const uint32_t waves = std::min<uint32_t>(p.n_lines, 4u * 2048u);
+        hipLaunchKernelGGL(k_line_raster, dim3((waves + 3u) / 4u), dim3(256), 0, s, p);
+    }
+    hipLaunchKernelGGL((k_line_resolve<SH>), dim3(blocks_of(npx, 256)), dim3(256), 0, s, p);
+}
+
+hipError_t launch_lines(const FrameParams &p, hipStream_t s) {
+    if ((size_t)p.v0.W * p.v0.H == 0) return hipSuccess;
+    switch (p.shader) {
+    case FLAT: launch_lines_t<FLAT>(p, s); break;
+    case COLOR: launch_lines_t<COLOR>(p, s); break;
+    case GOURAUD: launch_lines_t<GOURAUD>(p, s); break;
+    case PHONG: launch_lines_t<PHONG>(p, s); break;
+    case DEPTH_ONLY: launch_lines_t<DEPTH_ONLY>(p, s); break;
+    case TEXTURED: launch_lines_t<TEXTURED>(p, s); break;
+    case PHONG_SHADOW: launch_lines_t<PHONG_SHADOW>(p, s); break;
+    case TEX_SHADOW: launch_lines_t<TEX_SHADOW>(p, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -10,6 +10,7 @@
 #include <array>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -39,6 +40,14 @@ struct DevArray {
     size_t cap = 0;
 };
 
+// Device memory replaced while work already enqueued may still read it: freed
+// once `done` (recorded after that work) has completed.
+struct Deferred {
+    void *ptr;
+    size_t bytes;
+    hipEvent_t done;
+};
+
 }  // namespace
 
 struct euchip_ctx {
@@ -49,20 +58,30 @@ struct euchip_ctx {
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, Stat> stats;
-    // per-frame scratch, grown on demand
+    // per-frame scratch, grown on demand (old storage is freed once the frames
+    // enqueued on it have finished)
     DevArray<uint4> rec0, rec1;
     DevArray<float> vary;
-    DevArray<uint32_t> tile_count, bins;
-    DevArray<uint2> ovf;
+    DevArray<TileBin> tile_count;
+    DevArray<uint32_t> bins;
+    DevArray<uint32_t> pool;    // overflow segments
     DevArray<uint32_t> heavy;   // heavy-tile list
     DevArray<uint32_t> busy;    // busy-tile lists (one per XCD tile range)
     DevArray<uint32_t> big;     // big-triangle list
+    DevArray<uint32_t> lrec;    // line records
+    DevArray<uint64_t> lkey;    // per-pixel winner keys of a lines frame
+    std::vector<Deferred> deferred;
     uint32_t counts_used[2] = {0, 0};  // tiles of the last frame that counted into each half of tile_count
-    uint32_t tile_cap = 128;    // bin slots per tile; raised when a frame needed more
+    uint32_t tile_cap = 128;    // bin slots per tile; raised when a frame needed more (within the bin budget)
+    size_t pool_want = 0;       // pool words the next frames should have (from the reports)
     uint32_t frame = 0;
     uint32_t max_groups = 2048; // resident raster workgroups: CUs x 8
+    uint32_t big_tile_sx = 5;   // tiles of big-triangle frames: (1 << sx) x (256 >> sx) pixels
     Counters *counters = nullptr;
-    Counters *host_counters = nullptr;
+    Report *report = nullptr;   // host memory the raster pass writes each frame's counts to
+    Report *report_dev = nullptr;  // its device address
+    uint32_t seen_frame = ~0u;  // the last report taken into account
+    uint32_t seen_dropped = 0;  // frames reported dropped before the last synchronize
     size_t bytes_in_use = 0;    // device bytes held through this context
     size_t budget = 0;          // 0: no limit
 };
@@ -114,9 +133,11 @@ struct euchip_batch {
     size_t n_tiles = 0;                 // tiles of the launch (their counts are zeroed before setup)
     DevArray<uint4> rec0, rec1;
     DevArray<float> vary;
-    DevArray<uint32_t> tile_count, bins, ctrbuf;  // ctrbuf: Counters, then req_max per active request
+    DevArray<TileBin> tile_count;
+    DevArray<uint32_t> bins, ctrbuf;              // ctrbuf: Counters, then req_max per active request
     DevArray<uint32_t> busy;                      // busy-tile lists (one slot per tile)
-    DevArray<uint2> ovf;
+    DevArray<uint32_t> pool;                      // overflow segments
+    size_t pool_want = 0;                         // pool words the last launch asked for, plus a quarter
     uint32_t *host_ctr = nullptr;       // pinned copy of ctrbuf
     size_t host_ctr_words = 0;
     FrameParams p{};
@@ -174,20 +195,47 @@ void dev_free(euchip_ctx *ctx, void *ptr, size_t bytes) {
     ctx->bytes_in_use -= std::min(bytes, ctx->bytes_in_use);
 }
 
+// Free deferred storage whose frames have finished (all of it with `wait`).
+void reap_deferred(euchip_ctx *ctx, bool wait) {
+    size_t k = 0;
+    for (Deferred &d : ctx->deferred) {
+        if (wait) (void)hipEventSynchronize(d.done);
+        const bool done = wait || hipEventQuery(d.done) != hipErrorNotReady;
+        if (done) {
+            dev_free(ctx, d.ptr, d.bytes);
+            (void)hipEventDestroy(d.done);
+        } else {
+            ctx->deferred[k++] = d;
+        }
+    }
+    ctx->deferred.resize(k);
+}
+
 // Grow `a` to at least n elements (contents are not kept).  `s` is the stream
-// that may still use the old storage.
+// whose enqueued work may still use the old storage: it is freed once that
+// work has finished, without waiting for it here.
 template <class T>
 euchip_status ensure(euchip_ctx *ctx, DevArray<T> &a, size_t n, const char *what, hipStream_t s) {
     if (a.cap >= n && a.ptr) return EUCHIP_OK;
     size_t cap = std::max<size_t>(n, a.cap + a.cap / 2);
     cap = std::max<size_t>(cap, 256);
     if (a.ptr) {
-        HIP_TRY(ctx, hipStreamSynchronize(s));
-        dev_free(ctx, a.ptr, a.cap * sizeof(T));
+        hipEvent_t e = nullptr;
+        HIP_TRY(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        const hipError_t er = hipEventRecord(e, s);
+        if (er != hipSuccess) {
+            (void)hipEventDestroy(e);
+            return hip_fail(ctx, er, "hipEventRecord");
+        }
+        ctx->deferred.push_back({a.ptr, a.cap * sizeof(T), e});
         a.ptr = nullptr;
         a.cap = 0;
     }
     euchip_status st = dev_alloc(ctx, (void **)&a.ptr, cap * sizeof(T), what);
+    if (st == EUCHIP_OVER_BUDGET && !ctx->deferred.empty()) {  // storage on its way out counts too
+        reap_deferred(ctx, true);
+        st = dev_alloc(ctx, (void **)&a.ptr, cap * sizeof(T), what);
+    }
     if (st) return st;
     a.cap = cap;
     return EUCHIP_OK;
@@ -274,13 +322,36 @@ void fill_pipe(FrameParams &p, const euchip_pipeline_desc *d) {
     p.z_clip = d->z_clip ? 1u : 0u, p.z_min = d->z_min, p.z_max = d->z_max;
     p.cull = d->cull, p.msaa_level = d->aa_msaa ? std::min<uint32_t>(d->aa_level, 6u) : 0u;
     p.band_par = d->band_policy == EUCHIP_BANDS_PAR ? 1u : 0u;
+    p.bgra = d->pixel_format == EUCHIP_PIXEL_BGRA8 ? 1u : 0u;
+    p.sampler = d->sampler != EUCHIP_SAMPLER_DEFAULT ? d->sampler
+              : d->shader == EUCHIP_SHADER_TEX_SHADOW ? (uint32_t)EUCHIP_SAMPLER_LINEAR_CLAMPED
+                                                      : (uint32_t)EUCHIP_SAMPLER_LINEAR;
+    p.geometry = d->geometry;
+}
+
+// Triangles a stream of `count` elements per instance yields after the geometry
+// stage; for the line kinds, the lines (LineTriangleList: three per triangle).
+uint64_t stream_triangles(const euchip_pipeline_desc *d, uint64_t count, uint64_t instances) {
+    if (d->primitives == EUCHIP_PRIMS_LINE_LIST) return count * instances / 2;
+    const uint64_t tris = count * instances / 3 * (d->geometry == EUCHIP_GEOMETRY_BOTH_SIDES ? 2 : 1);
+    return d->primitives == EUCHIP_PRIMS_LINE_TRIANGLE_LIST ? 3 * tris : tris;
+}
+
+// Batches, servers and the no-allocation render draw triangle lists only.
+euchip_status check_triangles(euchip_ctx *ctx, const euchip_pipeline_desc *d) {
+    if (d->primitives != EUCHIP_PRIMS_TRIANGLE_LIST)
+        return fail(ctx, EUCHIP_INVALID_ARG, "line primitives are drawn by euchip_render / euchip_render_async only");
+    return EUCHIP_OK;
 }
 
 euchip_status check_desc(euchip_ctx *ctx, const euchip_pipeline_desc *d) {
     if (d->shader > EUCHIP_SHADER_TEX_SHADOW) return fail(ctx, EUCHIP_INVALID_ARG, "unknown shader %u", d->shader);
     if (d->depth_test > EUCHIP_DEPTH_GREATER || d->cull > EUCHIP_CULL_FRONT || d->band_policy > EUCHIP_BANDS_SEQ ||
-        d->xform > EUCHIP_XFORM_MVP)
+        d->xform > EUCHIP_XFORM_MVP || d->pixel_format > EUCHIP_PIXEL_BGRA8 || d->sampler > EUCHIP_SAMPLER_NEAREST_MIRRORED ||
+        d->geometry > EUCHIP_GEOMETRY_BOTH_SIDES || d->primitives > EUCHIP_PRIMS_LINE_TRIANGLE_LIST)
         return fail(ctx, EUCHIP_INVALID_ARG, "enum field out of range");
+    if (d->primitives == EUCHIP_PRIMS_LINE_LIST && d->geometry != EUCHIP_GEOMETRY_IDENTITY)
+        return fail(ctx, EUCHIP_INVALID_ARG, "the geometry stages act on triangles; a LineList takes the identity");
     return EUCHIP_OK;
 }
 
@@ -339,7 +410,7 @@ euchip_status plan_request(euchip_ctx *ctx, const euchip_pipeline_desc *d, const
             nothing = true;  // no worker threads spawned
         }
     }
-    P.n_tris = (uint64_t)mesh->count * mesh->n_instances / 3;
+    P.n_tris = stream_triangles(d, mesh->count, mesh->n_instances);
     P.n_sv = (uint64_t)mesh->n_instances * mesh->n_vertices;
     if (W == 0 || H == 0 || P.n_tris == 0) nothing = true;
     if (!nothing && (W >= (1u << 24) || H >= (1u << 24))) return fail(ctx, EUCHIP_INVALID_ARG, "target larger than 2^24");
@@ -385,21 +456,61 @@ euchip_status enqueue_clears(euchip_ctx *ctx, const euchip_pipeline_desc *d, con
     return EUCHIP_OK;
 }
 
-// Read back the counters after a frame and grow the bins; true when the frame
-// was dropped because the overflow list filled up.
-euchip_status after_frame(euchip_ctx *ctx, bool &dropped) {
-    const Counters &hc = *ctx->host_counters;
-    if (hc.max_count > ctx->tile_cap) {  // grow the bins for the frames that follow
+// Bin slots per tile of a frame: the capacity the reports asked for, within
+// the frame's bin budget of max(64 MiB, 64 bytes per triangle); the entries of
+// denser tiles go to their overflow segments.
+uint32_t frame_tile_cap(uint32_t want, uint64_t ntiles, uint64_t n_tris) {
+    const uint64_t budget = std::max<uint64_t>((uint64_t)64 << 20, 64 * n_tris);
+    uint32_t c = 16;
+    while (c < want && c < (1u << 24) && (uint64_t)(2 * c) * ntiles * 4 <= budget) c <<= 1;
+    return c;
+}
+
+// Pool words a frame starts with before any report: two per triangle, at least 2 Mi.
+size_t initial_pool_words(uint64_t n_tris) { return std::max<size_t>((size_t)2 << 20, 2 * n_tris); }
+
+// Take the raster pass's latest report into account (host memory it writes;
+// nothing waits for the stream): the bin capacity and the pool of the frames
+// that follow grow to what that frame needed.
+void absorb_report(euchip_ctx *ctx) {
+    const volatile Report *r = ctx->report;
+    const uint32_t f = r->frame;
+    if (f == ctx->seen_frame) return;
+    ctx->seen_frame = f;
+    const uint32_t mc = r->max_count, used = r->pool_used;
+    if (mc > ctx->tile_cap) {
         uint32_t c = ctx->tile_cap;
-        while (c < hc.max_count && c < (1u << 24)) c <<= 1;
+        while (c < mc && c < (1u << 24)) c <<= 1;
         ctx->tile_cap = c;
     }
-    dropped = (hc.flags & FLAG_BIN_OVERFLOW) != 0;
-    if (dropped) {
-        HIP_TRY(ctx, hipMemsetAsync(ctx->counters, 0, sizeof(Counters), ctx->stream));
-        HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4, ctx->stream));
+    ctx->pool_want = std::max<size_t>(ctx->pool_want, (size_t)used + used / 4);
+}
+
+// render() of one request of lines (lines.rs:15-121) on the context's stream.
+euchip_status render_lines(euchip_ctx *ctx, const euchip_pipeline_desc *d, const Plan &P, bool sync) {
+    euchip_status s;
+    FrameParams p;
+    memset(&p, 0, sizeof(p));
+    fill_pipe(p, d);
+    p.v0 = P.v;
+    p.n_views = 1;
+    p.primitives = d->primitives;
+    if (P.n_tris >= (1ull << 31)) return fail(ctx, EUCHIP_INVALID_ARG, "too many lines");
+    p.n_lines = (uint32_t)P.n_tris;
+    const size_t npx = (size_t)P.v.W * P.v.H;
+    if ((s = ensure(ctx, ctx->vary, std::max<size_t>(P.n_sv * vd_count(d->shader), 1), "vertex data", ctx->stream)))
+        return s;
+    if ((s = ensure(ctx, ctx->lrec, std::max<size_t>((size_t)p.n_lines * kLineRecWords, 1), "line records",
+                    ctx->stream)))
+        return s;
+    if ((s = ensure(ctx, ctx->lkey, npx, "line keys", ctx->stream))) return s;
+    p.vary = ctx->vary.ptr;
+    p.lrec = ctx->lrec.ptr;
+    p.lkey = ctx->lkey.ptr;
+    if ((s = timed(ctx, "lines", [&] { return launch_lines(p, ctx->stream); }))) return s;
+    if (sync) {
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        ctx->counts_used[0] = ctx->counts_used[1] = 0;
+        if (ctx->profiling) resolve_pending(ctx);
     }
     return EUCHIP_OK;
 }
@@ -417,12 +528,23 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         if (sync) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         return P.status;
     }
+    reap_deferred(ctx, false);
+    if (d->primitives != EUCHIP_PRIMS_TRIANGLE_LIST) return render_lines(ctx, d, P, sync);
+    absorb_report(ctx);
     FrameParams p;
     memset(&p, 0, sizeof(p));
     fill_pipe(p, d);
     p.tile_sx = 5;
     p.v0 = P.v;
     tile_grid(p.v0, p.tile_sx);
+    // big-triangle mode when the mesh has no more triangles than the target has
+    // 32 x 8 tiles (triangles then tend to cover many tiles each); its tiles are
+    // wider (fewer tiles per row walk the same row start)
+    const bool big_mode = P.n_tris <= (uint64_t)p.v0.ntx * p.v0.nty;
+    if (big_mode && ctx->big_tile_sx != p.tile_sx) {
+        p.tile_sx = ctx->big_tile_sx;
+        tile_grid(p.v0, p.tile_sx);
+    }
     const size_t ntiles = (size_t)p.v0.ntx * p.v0.nty;
     p.ntiles = (uint32_t)ntiles;
     p.n_views = 1;
@@ -436,39 +558,38 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     const bool fresh_tiles = ctx->tile_count.cap < 2 * ntiles;
     if ((s = ensure(ctx, ctx->tile_count, 2 * ntiles, "tile counts", ctx->stream))) return s;
     if (fresh_tiles) {
-        HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * 4, ctx->stream));
+        HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * sizeof(TileBin), ctx->stream));
         ctx->counts_used[0] = ctx->counts_used[1] = 0;
     }
     const size_t half = ctx->tile_count.cap / 2;
-    if ((s = ensure(ctx, ctx->ovf, std::max<size_t>((size_t)1 << 20, (size_t)P.n_tris), "overflow list", ctx->stream)))
-        return s;
     if ((s = ensure(ctx, ctx->heavy, ntiles, "heavy tiles", ctx->stream))) return s;
     if ((s = ensure(ctx, ctx->busy, 8 * ntiles, "busy tiles", ctx->stream))) return s;
     const size_t big_cap = std::min<size_t>(std::max<size_t>(P.n_tris, 1), kBigCap);
     if ((s = ensure(ctx, ctx->big, big_cap, "big triangles", ctx->stream))) return s;
 
     for (int attempt = 0;; attempt++) {
-        if ((s = ensure(ctx, ctx->bins, ntiles * ctx->tile_cap, "bins", ctx->stream))) return s;
+        const uint32_t cap = frame_tile_cap(ctx->tile_cap, ntiles, P.n_tris);
+        if ((s = ensure(ctx, ctx->bins, ntiles * cap, "bins", ctx->stream))) return s;
+        const size_t pool_words = std::min<size_t>(std::max(ctx->pool_want, initial_pool_words(P.n_tris)), 0xFFFFFFF0ull);
+        if ((s = ensure(ctx, ctx->pool, pool_words, "overflow pool", ctx->stream))) return s;
         p.rec0 = ctx->rec0.ptr;
         p.rec1 = ctx->rec1.ptr;
         p.vary = ctx->vary.ptr;
         p.ctr = ctx->counters;
-        p.req_max = &ctx->counters->max_count;
+        p.report = ctx->report_dev;
         const uint32_t par = ctx->frame & 1u, npar = par ^ 1u;
         p.tile_count = ctx->tile_count.ptr + par * half;
         p.count_next = ctx->tile_count.ptr + npar * half;
         p.zero_n = ctx->counts_used[npar];
         p.busy = ctx->busy.ptr;
         p.busy_map = BUSY_BLOCKS;
-        // big-triangle mode when the mesh has no more triangles than the target
-        // has tiles (triangles then tend to cover many tiles each)
         p.big = ctx->big.ptr;
-        p.big_cap = P.n_tris <= ntiles ? (uint32_t)std::min<size_t>(ctx->big.cap, kBigCap) : 0u;
+        p.big_cap = big_mode ? (uint32_t)std::min<size_t>(ctx->big.cap, kBigCap) : 0u;
         p.bins = ctx->bins.ptr;
-        p.v0.tile_cap = ctx->tile_cap;
+        p.v0.tile_cap = cap;
         p.v0.bin_base = 0;
-        p.ovf = ctx->ovf.ptr;
-        p.ovf_cap = (uint32_t)std::min<size_t>(ctx->ovf.cap, 0xFFFFFFF0ull);
+        p.pool = ctx->pool.ptr;
+        p.pool_cap = (uint32_t)std::min<size_t>(ctx->pool.cap, 0xFFFFFFF0ull);
         p.heavy = ctx->heavy.ptr;
         p.heavy_cap = (uint32_t)std::min<size_t>(ctx->heavy.cap, 0xFFFFFFF0ull);
         p.frame = ctx->frame++;
@@ -477,16 +598,15 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         ctx->counts_used[npar] = 0;
         ctx->counts_used[par] = (uint32_t)ntiles;
         if (!sync) return EUCHIP_OK;
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->host_counters, ctx->counters, sizeof(Counters), hipMemcpyDeviceToHost,
-                                    ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->profiling) resolve_pending(ctx);
-        bool dropped = false;
-        if ((s = after_frame(ctx, dropped))) return s;
-        if (!dropped) return EUCHIP_OK;
-        // The overflow list filled up: nothing was written.  Retry larger.
+        absorb_report(ctx);  // this frame's
+        const volatile Report *r = ctx->report;
+        if (!(r->flags & FLAG_BIN_OVERFLOW)) return EUCHIP_OK;
+        // The pool ran out: nothing was written.  Re-run with the pool the
+        // frame asked for (absorb_report raised pool_want to it).
+        ctx->seen_dropped = r->dropped;
         if (attempt >= 3) return fail(ctx, EUCHIP_OVER_BUDGET, "bin storage overflow");
-        if ((s = ensure(ctx, ctx->ovf, ctx->ovf.cap * 4, "overflow list", ctx->stream))) return s;
     }
 }
 
@@ -588,7 +708,8 @@ euchip_status build_tables(euchip_batch *b) {
     if ((s = ensure(ctx, b->busy, std::max<size_t>(tiles, 1), "batch busy tiles", b->stream))) return s;
     b->n_tiles = tiles;
     if ((s = ensure(ctx, b->bins, std::max<size_t>(bins, 1), "batch bins", b->stream))) return s;
-    if ((s = ensure(ctx, b->ovf, std::max<size_t>((size_t)1 << 20, tris), "batch overflow list", b->stream))) return s;
+    if ((s = ensure(ctx, b->pool, std::max(b->pool_want, initial_pool_words(tris)), "batch overflow pool", b->stream)))
+        return s;
     const size_t words = sizeof(Counters) / 4 + na;
     if ((s = ensure(ctx, b->ctrbuf, words, "batch counters", b->stream))) return s;
     if (b->host_ctr_words < words) {
@@ -617,8 +738,9 @@ euchip_status build_tables(euchip_batch *b) {
     p.req_max = b->ctrbuf.ptr + sizeof(Counters) / 4;
     p.tile_count = b->tile_count.ptr;
     p.bins = b->bins.ptr;
-    p.ovf = b->ovf.ptr;
-    p.ovf_cap = (uint32_t)std::min<size_t>(b->ovf.cap, 0xFFFFFFF0ull);
+    p.pool = b->pool.ptr;
+    p.pool_cap = (uint32_t)std::min<size_t>(b->pool.cap, 0xFFFFFFF0ull);
+    p.report = nullptr;
     p.heavy = nullptr;  // batches keep their tile order: most of their tiles would be heavy
     p.heavy_cap = 0;
     p.busy = b->busy.ptr;
@@ -659,7 +781,8 @@ euchip_status enqueue_batch(euchip_batch *b, bool capturing = false) {
     if (b->active.empty()) return EUCHIP_OK;
     const size_t words = sizeof(Counters) / 4 + b->active.size();
     HIP_TRY(ctx, launch_fill_u32(b->ctrbuf.ptr, 0u, words, b->stream));
-    HIP_TRY(ctx, launch_fill_u32(b->tile_count.ptr, 0u, b->n_tiles, b->stream));  // raster leaves them as it read them
+    HIP_TRY(ctx, launch_fill_u32(reinterpret_cast<uint32_t *>(b->tile_count.ptr), 0u, 2 * b->n_tiles,
+                                 b->stream));  // raster leaves them as it read them
     euchip_status s;
     if (ctx->profiling && !capturing) {
         if ((s = timed_on(ctx, b->stream, "batch_setup", [&] { return launch_setup(b->p, b->stream, true, b->setup_blocks); })))
@@ -717,15 +840,20 @@ euchip_status wait_batch(euchip_batch *b) {
         std::vector<uint32_t> &caps = b->caps[b->tile_sx - 3];
         for (size_t k = 0; k < b->active.size(); k++) {
             if (req_max[k] > caps[k]) {
-                caps[k] = pow2_at_least(req_max[k]);
-                b->tables_dirty = true;
+                const Plan &P = b->plans[b->active[k]];
+                View v = P.v;
+                tile_grid(v, b->tile_sx);
+                const uint32_t c = frame_tile_cap(req_max[k], (uint64_t)v.ntx * v.nty, P.n_tris);
+                if (c > caps[k]) caps[k] = c, b->tables_dirty = true;
             }
         }
-        if (!(hc.flags & FLAG_BIN_OVERFLOW)) return EUCHIP_OK;
+        const size_t asked = hc.pool[0][0];
+        b->pool_want = std::max<size_t>(b->pool_want, asked + asked / 4);
+        if (!(hc.flags[0][0] & FLAG_BIN_OVERFLOW)) return EUCHIP_OK;
+        // the pool ran out: nothing was written; re-run with what it asked for
         if (attempt >= 3) return fail(ctx, EUCHIP_OVER_BUDGET, "batch bin storage overflow");
-        euchip_status s;
-        if ((s = ensure(ctx, b->ovf, b->ovf.cap * 4, "batch overflow list", b->stream))) return s;
         b->tables_dirty = true;
+        euchip_status s;
         if ((s = launch_batch(b, false))) return s;
     }
 }
@@ -744,7 +872,7 @@ void free_batch_memory(euchip_batch *b) {
     release(ctx, b->busy);
     release(ctx, b->bins);
     release(ctx, b->ctrbuf);
-    release(ctx, b->ovf);
+    release(ctx, b->pool);
     if (b->host_ctr) (void)hipHostFree(b->host_ctr);
     b->host_ctr = nullptr;
     b->host_ctr_words = 0;
@@ -755,11 +883,11 @@ void free_batch_memory(euchip_batch *b) {
 namespace euchip {
 
 size_t request_scratch_estimate(const euchip_pipeline_desc *d, const euchip_mesh *m, uint32_t W, uint32_t H) {
-    const uint64_t n_tris = (uint64_t)m->count * m->n_instances / 3;
+    const uint64_t n_tris = stream_triangles(d, m->count, m->n_instances);
     const uint64_t n_sv = (uint64_t)m->n_instances * m->n_vertices;
     const uint64_t nt = (uint64_t)((W + 31) / 32) * ((H + 7) / 8);
     const uint64_t nb = (std::max<uint64_t>(n_sv, n_tris) + 255) / 256;
-    return sizeof(View) + nb * sizeof(uint2) + nt * 12 + n_sv * 4 * vd_count(d->shader) +
+    return sizeof(View) + nb * sizeof(uint2) + nt * 16 + n_sv * 4 * vd_count(d->shader) +
            n_tris * 16 * (kRec0Quads + kRec1Quads) + nt * initial_cap(n_tris, nt) * 4 + 4 + 8 * n_tris;
 }
 
@@ -797,16 +925,24 @@ euchip_status euchip_ctx_create(int device, euchip_ctx **out) {
     if (hipSetDevice(device) != hipSuccess) return EUCHIP_HIP_ERROR;
     euchip_ctx *ctx = new euchip_ctx();
     ctx->device = device;
+    if (const char *e = getenv("EUCHIP_BIG_TILE_SX")) {  // tuning: tile shape of big-triangle frames
+        const int v = atoi(e);
+        if (v >= 3 && v <= 8) ctx->big_tile_sx = (uint32_t)v;
+    }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         ctx->max_groups = (uint32_t)cus * 8u;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void **)&ctx->counters, sizeof(Counters)) != hipSuccess ||
-        hipHostMalloc((void **)&ctx->host_counters, sizeof(Counters), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->report, sizeof(Report), hipHostMallocDefault) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&ctx->report_dev, ctx->report, 0) != hipSuccess ||
         hipMemset(ctx->counters, 0, sizeof(Counters)) != hipSuccess) {
         euchip_ctx_destroy(ctx);
         return EUCHIP_HIP_ERROR;
     }
+    memset(ctx->report, 0, sizeof(Report));
+    ctx->report->frame = ~0u;  // no frame reported yet
+
     *out = ctx;
     return EUCHIP_OK;
 }
@@ -825,13 +961,16 @@ euchip_status euchip_ctx_destroy(euchip_ctx *ctx) {
     release(ctx, ctx->vary);
     release(ctx, ctx->tile_count);
     release(ctx, ctx->bins);
-    release(ctx, ctx->ovf);
+    release(ctx, ctx->pool);
     release(ctx, ctx->heavy);
     release(ctx, ctx->busy);
     release(ctx, ctx->big);
+    release(ctx, ctx->lrec);
+    release(ctx, ctx->lkey);
 
+    reap_deferred(ctx, true);
     (void)hipFree(ctx->counters);
-    if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
+    if (ctx->report) (void)hipHostFree(ctx->report);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return EUCHIP_OK;
@@ -844,14 +983,18 @@ void *euchip_ctx_stream(euchip_ctx *ctx) { return ctx ? (void *)ctx->stream : nu
 euchip_status euchip_synchronize(euchip_ctx *ctx) {
     if (!ctx) return EUCHIP_INVALID_ARG;
     (void)hipSetDevice(ctx->device);
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->host_counters, ctx->counters, sizeof(Counters), hipMemcpyDeviceToHost,
-                                ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->profiling) resolve_pending(ctx);
-    bool dropped = false;
-    euchip_status s = after_frame(ctx, dropped);
-    if (s) return s;
-    if (dropped) return fail(ctx, EUCHIP_OVER_BUDGET, "bin storage overflow in an asynchronous frame; it was dropped");
+    reap_deferred(ctx, false);
+    absorb_report(ctx);
+    const uint32_t dropped = ((const volatile Report *)ctx->report)->dropped;
+    if (dropped != ctx->seen_dropped) {
+        const uint32_t n = dropped - ctx->seen_dropped;
+        ctx->seen_dropped = dropped;
+        return fail(ctx, EUCHIP_OVER_BUDGET,
+                    "%u asynchronous frame(s) ran out of bin overflow storage and were not written; the frames "
+                    "enqueued after this call use storage of the size they asked for", n);
+    }
     return EUCHIP_OK;
 }
 
@@ -1060,6 +1203,7 @@ euchip_status euchip_batch_set(euchip_batch *b, const euchip_pipeline_desc *desc
     euchip_ctx *ctx = b->ctx;
     (void)hipSetDevice(ctx->device);
     euchip_status s = check_desc(ctx, desc);
+    if (!s) s = check_triangles(ctx, desc);
     if (s) return s;
     HIP_TRY(ctx, hipStreamSynchronize(b->stream));
     std::vector<Plan> plans(n);
@@ -1170,7 +1314,8 @@ size_t euchip_batch_scratch_bytes(const euchip_batch *b) {
     if (!b) return 0;
     return b->tables.cap * 16 + b->rec0.cap * 16 +
            b->rec1.cap * 16 +
-           b->vary.cap * 4 + b->tile_count.cap * 4 + b->busy.cap * 4 + b->bins.cap * 4 + b->ctrbuf.cap * 4 + b->ovf.cap * 8;
+           b->vary.cap * 4 + b->tile_count.cap * sizeof(TileBin) + b->busy.cap * 4 + b->bins.cap * 4 + b->ctrbuf.cap * 4 +
+           b->pool.cap * 4;
 }
 
 size_t euchip_batch_estimate_bytes(const euchip_pipeline_desc *desc, const euchip_request *reqs, uint32_t n) {

@@ -301,6 +301,10 @@ euchip_status euchip_server_create(euchip_ctx *ctx, const euchip_pipeline_desc *
     // The server owns its targets and hands out pooled storage that still holds
     // an earlier request's pixels and depth: every request must start from the
     // desc's clear values, or it would see (and depth-test against) another's.
+    if (desc->primitives != EUCHIP_PRIMS_TRIANGLE_LIST) {
+        euchip::set_last_error(ctx, "servers draw triangle lists");
+        return EUCHIP_INVALID_ARG;
+    }
     if (!desc->clear_color_enable || !desc->clear_depth_enable) {
         euchip::set_last_error(ctx, "server pipelines must enable both clears (clear_color_enable, clear_depth_enable): "
                                    "pooled targets hold earlier requests' results");

@@ -127,6 +127,42 @@ EUCHIP_DEV void sample_bilinear(const void *data, uint32_t w, uint32_t h, float 
     }
 }
 
+// f32::rem_euclid: x % m (exact), plus |m| when negative.
+EUCHIP_DEV float rem_euclid(float x, float m) {
+    const float r = fmodf(x, m);
+    return r < 0.0f ? r + fabsf(m) : r;
+}
+
+// Texture sample through sampler `kind` (EUCHIP_SAMPLER_LINEAR ..
+// NEAREST_MIRRORED): the wrapper's index map (sampler/mod.rs:98-179: none,
+// Clamped, Tiled, Mirrored), then Linear or Nearest (sampler/nearest.rs:20-38
+// with Denormalize f32 -> usize, math.rs:52-55) of an RGBA8 texture.
+EUCHIP_DEV void sample_rgba8(const uint32_t *tex, uint32_t w, uint32_t h, uint32_t kind, float x, float y, float out[4]) {
+    const bool nearest = kind >= SAMPLER_NEAREST;
+    const uint32_t wrap = nearest ? kind - SAMPLER_NEAREST : kind - SAMPLER_LINEAR;
+    if (wrap == 1u) {
+        x = rmin(rmax(x, 0.0f), 1.0f);
+        y = rmin(rmax(y, 0.0f), 1.0f);
+    } else if (wrap == 2u) {
+        x = rem_euclid(x, 1.0f);
+        y = rem_euclid(y, 1.0f);
+    } else if (wrap == 3u) {
+        x = rem_euclid(x, 2.0f) >= 1.0f ? 1.0f - rem_euclid(x, 1.0f) : rem_euclid(x, 1.0f);
+        y = rem_euclid(y, 2.0f) >= 1.0f ? 1.0f - rem_euclid(y, 1.0f) : rem_euclid(y, 1.0f);
+    }
+    if (!nearest) {
+        sample_bilinear<4>(tex, w, h, x, y, false, out);
+        return;
+    }
+    // ((u * size as f32).max(0.0) as usize).min(size - 1)
+    const float fx = rmax(x * (float)w, 0.0f), fy = rmax(y * (float)h, 0.0f);
+    const uint32_t px = fx >= (float)w ? w - 1u : ((uint32_t)fx < w - 1u ? (uint32_t)fx : w - 1u);
+    const uint32_t py = fy >= (float)h ? h - 1u : ((uint32_t)fy < h - 1u ? (uint32_t)fy : h - 1u);
+    const uint32_t q = tex[px + w * py];
+#pragma unroll
+    for (int c = 0; c < 4; c++) out[c] = (float)((q >> (8 * c)) & 0xffu);
+}
+
 // ---------------------------------------------------------------------------
 // Pipeline::vertex of the fixed shader set
 // ---------------------------------------------------------------------------
@@ -197,7 +233,7 @@ EUCHIP_DEV bool in_light(const View &v, const float lvp[3]) {
 
 // Pipeline::fragment of the fixed shader set.  out: Rgba<f32> (FLAT: colour bits in out[0]).
 template <uint32_t SH>
-EUCHIP_DEV void fragment_shader(const View &v, const float *vd, float out[4]) {
+EUCHIP_DEV void fragment_shader(const View &v, uint32_t sampler, const float *vd, float out[4]) {
     const float *u = v.u;
     out[0] = out[1] = out[2] = out[3] = 0.0f;
     if constexpr (SH == FLAT) {
@@ -207,7 +243,7 @@ EUCHIP_DEV void fragment_shader(const View &v, const float *vd, float out[4]) {
     } else if constexpr (SH == PHONG) {
         lighting2(u, vd, vd + 3, out);
     } else if constexpr (SH == TEXTURED) {
-        sample_bilinear<4>(v.tex, v.tex_w, v.tex_h, vd[0], vd[1], false, out);
+        sample_rgba8(v.tex, v.tex_w, v.tex_h, sampler, vd[0], vd[1], out);
     } else if constexpr (SH == PHONG_SHADOW) {
         float wn[3], cd[3], ld[3], t3[3];
         normalize3(vd + 3, wn);
@@ -228,7 +264,7 @@ EUCHIP_DEV void fragment_shader(const View &v, const float *vd, float out[4]) {
         normalize3(vd + 3, n);
         float ndl = rmax(dot3s(n, u + U_L0), 0.0f);
         float light = u[U_K + 0] + (in_light(v, vd + 8) ? ndl * u[U_K + 1] : 0.0f);
-        sample_bilinear<4>(v.tex, v.tex_w, v.tex_h, vd[6], vd[7], true, texc);
+        sample_rgba8(v.tex, v.tex_w, v.tex_h, sampler, vd[6], vd[7], texc);
         float s = light * (1.0f / 255.0f);
         out[0] = texc[0] * s;
         out[1] = texc[1] * s;
@@ -239,7 +275,7 @@ EUCHIP_DEV void fragment_shader(const View &v, const float *vd, float out[4]) {
 
 // Pipeline::blend: every fixed shader replaces the old pixel (DEPTH_ONLY's Unit keeps it).
 template <uint32_t SH>
-EUCHIP_DEV uint32_t blend_shader(uint32_t old, const float f[4]) {
+EUCHIP_DEV uint32_t blend_shader(uint32_t old, const float f[4], uint32_t bgra) {
     if constexpr (SH == FLAT) return __float_as_uint(f[0]);
     if constexpr (SH == DEPTH_ONLY) return old;
     uint32_t b[4];
@@ -250,6 +286,7 @@ EUCHIP_DEV uint32_t blend_shader(uint32_t old, const float f[4]) {
 #pragma unroll
         for (int c = 0; c < 4; c++) b[c] = sat_u8(rmin(rmax(f[c], 0.0f), 1.0f) * 255.0f);
     }
+    if (bgra) return b[2] | (b[1] << 8) | (b[0] << 16) | (b[3] << 24);
     return b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
 }
 

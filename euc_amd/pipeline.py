@@ -53,6 +53,40 @@ class Xform(enum.IntEnum):
     MVP = 1   # clip = view_proj * (model[instance] * (p, 1))
 
 
+class PixelFormat(enum.IntEnum):
+    """How the Rgba<f32> shaders' blend packs a pixel into the u32 texel."""
+    RGBA8 = 0  # u32::from_le_bytes([r, g, b, a])
+    BGRA8 = 1  # u32::from_le_bytes([b, g, r, a]) (examples/teapot.rs:135-140, a window framebuffer)
+
+
+class Sampler(enum.IntEnum):
+    """Texture sampler of TEXTURED / TEX_SHADOW (src/sampler/*.rs): Linear or
+    Nearest, bare or wrapped in Clamped / Tiled / Mirrored."""
+    DEFAULT = 0          # TEXTURED: Linear; TEX_SHADOW: Clamped<Linear>
+    LINEAR = 1
+    LINEAR_CLAMPED = 2
+    LINEAR_TILED = 3
+    LINEAR_MIRRORED = 4
+    NEAREST = 5
+    NEAREST_CLAMPED = 6
+    NEAREST_TILED = 7
+    NEAREST_MIRRORED = 8
+
+
+class Geometry(enum.IntEnum):
+    """Fixed Pipeline::geometry stages (src/pipeline.rs:221-229) of a triangle [a, b, c]."""
+    IDENTITY = 0    # output([a, b, c]) -- the default
+    REVERSED = 1    # output([a, c, b]): the other winding
+    BOTH_SIDES = 2  # output([a, b, c]); output([a, c, b])
+
+
+class Primitives(enum.IntEnum):
+    """PrimitiveKind of the vertex stream (src/primitives.rs)."""
+    TRIANGLE_LIST = 0
+    LINE_LIST = 1
+    LINE_TRIANGLE_LIST = 2
+
+
 @dataclass(frozen=True)
 class DepthMode:
     test: Ordering = Ordering.NONE
@@ -125,11 +159,16 @@ class PipelineDesc:
     aa_level: int = 0
     band_policy: int = BandPolicy.PAR
     threads: int = 8
+    pixel_format: int = PixelFormat.RGBA8
+    sampler: int = Sampler.DEFAULT
+    geometry: int = Geometry.IDENTITY
+    primitives: int = Primitives.TRIANGLE_LIST
 
     @staticmethod
     def make(shader=Shader.FLAT, xform=Xform.MVP, pixel_mode=PixelMode.WRITE, depth_mode=DepthMode.NONE,
              coordinate_mode=CoordinateMode.VULKAN, cull=CullMode.BACK, aa_mode=AaMode.NONE,
-             band_policy=BandPolicy.PAR, threads=8) -> "PipelineDesc":
+             band_policy=BandPolicy.PAR, threads=8, pixel_format=PixelFormat.RGBA8, sampler=Sampler.DEFAULT,
+             geometry=Geometry.IDENTITY, primitives=Primitives.TRIANGLE_LIST) -> "PipelineDesc":
         zc = coordinate_mode.z_clip_range
         return PipelineDesc(
             shader=int(shader), xform=int(xform), pixel_write=pixel_mode.write,
@@ -137,7 +176,8 @@ class PipelineDesc:
             y_up=coordinate_mode.y_up, z_clip=zc is not None,
             z_min=0.0 if zc is None else float(zc[0]), z_max=0.0 if zc is None else float(zc[1]),
             cull=int(cull), aa_msaa=aa_mode.msaa, aa_level=int(aa_mode.level),
-            band_policy=int(band_policy), threads=threads)
+            band_policy=int(band_policy), threads=threads, pixel_format=int(pixel_format), sampler=int(sampler),
+            geometry=int(geometry), primitives=int(primitives))
 
     def msaa_level(self) -> int:
         """pipeline.rs:291-294: level.max(0).min(6), 0 for AaMode::None."""

@@ -136,7 +136,9 @@ def desc_c(desc: PipelineDesc, texture=None, shadow=None, clear_color=None, clea
         clear_color=0 if clear_color is None else int(clear_color) & 0xFFFFFFFF,
         clear_depth_enable=int(clear_depth is not None),
         clear_depth=0.0 if clear_depth is None else float(clear_depth),
-        texture=texture.handle if texture else None, shadow=shadow.handle if shadow else None)
+        texture=texture.handle if texture else None, shadow=shadow.handle if shadow else None,
+        pixel_format=int(desc.pixel_format), sampler=int(desc.sampler), geometry=int(desc.geometry),
+        primitives=int(desc.primitives))
 
 
 class Request:

@@ -110,6 +110,19 @@ def perspective_fov_lh_zo(fov_y, width, height, near, far) -> np.ndarray:
     return m
 
 
+def perspective_fov_rh_no(fov_y, width, height, near, far) -> np.ndarray:
+    """Right-handed perspective with z in [-1, 1] (examples/texture_mapping.rs:123)."""
+    h = np.cos(0.5 * fov_y) / np.sin(0.5 * fov_y)
+    w = h * height / width
+    m = np.zeros((4, 4))
+    m[0, 0] = w
+    m[1, 1] = h
+    m[2, 2] = -(far + near) / (far - near)
+    m[2, 3] = -(2.0 * far * near) / (far - near)
+    m[3, 2] = -1.0
+    return m
+
+
 def look_at_lh(eye, target, up) -> np.ndarray:
     eye, target, up = (np.asarray(a, dtype=np.float64) for a in (eye, target, up))
     f = target - eye
@@ -272,6 +285,57 @@ def torus_scene(seed=0x5EED0000 + 2, width=1920, height=1080, nu=256, nv=128, in
                  cam=(0.0, 0.0, 0.0), surf=(1.0, 0.8, 0.7, 1.0), ambient=0.1, k_diffuse=0.5, k_specular=3.0)
     mesh = Mesh(pos=pos, nrm=nrm, idx=idx, count=int(idx.size), n_instances=instances, models=models)
     return Scene("torus", mesh, u, width, height, 0, 1.0)
+
+
+def teapot_scene(mesh: Mesh, width: int, height: int, shadow_size: int = 512) -> Scene:
+    """The scene of the reference's benchmark (benches/teapot.rs:143-205) for a
+    loaded teapot mesh: a 512^2 shadow map from a light at (-8, 5, -5) looking at
+    the origin (perspective fov 0.75, near 0.1, far 100), then the Phong +
+    shadow-lookup colour pass from a camera 4.5 units away (fov 1.3, near 0.01,
+    far 100); model = rotation_x(pi) rotation_x(-0.55) rotation_y(-0.25).
+    Uniforms of the colour pass (PHONG_SHADOW) in `uniforms`, of the shadow pass
+    (DEPTH_ONLY) in extra["shadow_uniforms"]; the mesh carries the model matrix."""
+    light_pos = np.array([-8.0, 5.0, -5.0])
+    light_vp = perspective_fov_lh_zo(0.75, shadow_size, shadow_size, 0.1, 100.0) @ \
+        look_at_lh(light_pos, (0.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+    p = perspective_fov_lh_zo(1.3, width, height, 0.01, 100.0)
+    v = translation(0.0, 0.0, 4.5)
+    m = rotation(np.pi, 0.0, 0.0) @ rotation(-0.55, 0.0, 0.0) @ rotation(0.0, -0.25, 0.0)
+    cam = np.linalg.inv(v) @ np.array([0.0, 0.0, 0.0, 1.0])
+    u = uniforms(view_proj=f32(p @ v), cam=tuple(cam[:3]), surf=(1.0, 0.8, 0.7, 1.0), ambient=0.1, k_diffuse=0.5,
+                 k_specular=3.0, shadow_bias=0.0001, light_pos=tuple(light_pos), light_vp=f32(light_vp))
+    tm = Mesh(pos=mesh.pos, nrm=mesh.nrm, uv=mesh.uv, idx=mesh.idx, count=mesh.count, n_instances=1,
+              models=f32(m).reshape(1, 16))
+    sc = Scene("teapot", tm, u, width, height, 0, 1.0)
+    sc.extra["shadow_size"] = shadow_size
+    sc.extra["shadow_uniforms"] = uniforms(view_proj=f32(light_vp))
+    return sc
+
+
+CUBE_CORNERS = [(-1, -1, 1), (-1, 1, 1), (1, 1, 1), (1, -1, 1), (-1, -1, -1), (-1, 1, -1), (1, 1, -1), (1, -1, -1),
+                (-1, 1, 1), (-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, -1, 1), (-1, -1, -1), (1, -1, -1), (1, -1, 1),
+                (1, -1, 1), (1, -1, -1), (1, 1, -1), (1, 1, 1), (-1, -1, 1), (-1, -1, -1), (-1, 1, -1), (-1, 1, 1)]
+CUBE_UVS = [(0, 1), (0, 0), (1, 0), (1, 1), (0, 0), (0, 1), (1, 1), (1, 0), (0, 0), (0, 1), (1, 1), (1, 0),
+            (0, 0), (0, 1), (1, 1), (1, 0), (1, 1), (1, 0), (0, 0), (0, 1), (0, 1), (0, 0), (1, 0), (1, 1)]
+CUBE_INDICES = [0, 3, 1, 1, 3, 2, 4, 5, 7, 5, 6, 7, 8, 11, 9, 9, 11, 10, 12, 13, 15, 13, 14, 15,
+                16, 17, 19, 17, 18, 19, 20, 23, 21, 21, 23, 22]
+
+
+def texture_cube_scene(texture: np.ndarray, frame: int = 0, width: int = 800, height: int = 600) -> Scene:
+    """examples/texture_mapping.rs: a cube of 24 corners and 12 triangles with
+    per-face texture coordinates, mvp = p * v * m at animation frame `frame`, the
+    colour target cleared to 180 (TEXTURED, bare Linear sampler, no depth test)."""
+    p = perspective_fov_rh_no(1.4, width, height, 0.01, 100.0)
+    v = translation(0.0, 0.0, -2.0) @ scaling(0.6) @ rotation(0.6, 0.0, 0.0)
+    i = float(frame)
+    m = rotation(np.sin(i * 0.004) * 0.4, 0.0, 0.0) @ rotation(0.0, i * 0.0008 * 4.0, 0.0) @ \
+        rotation(0.0, 0.0, np.cos(i * 0.006) * 0.4)
+    pos = np.array([c + (1.0,) for c in CUBE_CORNERS], np.float32)
+    uv = np.array(CUBE_UVS, np.float32)
+    mesh = Mesh(pos=pos[:, :3].copy(), uv=uv, idx=np.array(CUBE_INDICES, np.uint32), count=len(CUBE_INDICES),
+                n_instances=1, models=np.eye(4, dtype=np.float32).reshape(1, 16))
+    u = uniforms(view_proj=f32(p @ v @ m))
+    return Scene("texture_cube", mesh, u, width, height, 180, 1.0, texture=texture)
 
 
 def value_noise(seed, size, cells):

@@ -65,6 +65,29 @@ typedef enum { EUCHIP_CULL_NONE = 0, EUCHIP_CULL_BACK = 1, EUCHIP_CULL_FRONT = 2
 typedef enum { EUCHIP_BANDS_PAR = 0, EUCHIP_BANDS_SEQ = 1 } euchip_band_policy;
 /* How the vertex shader forms the clip position. */
 typedef enum { EUCHIP_XFORM_CLIP = 0, EUCHIP_XFORM_MVP = 1 } euchip_xform;
+/* How the blend of the Rgba<f32> shaders packs a pixel into its u32 texel:
+ * u32::from_le_bytes([r, g, b, a]), or [b, g, r, a] as the teapot example's
+ * window framebuffer does (examples/teapot.rs:135-140). */
+typedef enum { EUCHIP_PIXEL_RGBA8 = 0, EUCHIP_PIXEL_BGRA8 = 1 } euchip_pixel_format;
+/* Texture sampler of TEXTURED / TEX_SHADOW: Linear (src/sampler/linear.rs:30-66)
+ * or Nearest (src/sampler/nearest.rs:20-38, Denormalize src/math.rs:43-79),
+ * bare or wrapped in Clamped / Tiled / Mirrored (src/sampler/mod.rs:98-179).
+ * DEFAULT: Linear for TEXTURED, Clamped<Linear> for TEX_SHADOW. */
+typedef enum {
+    EUCHIP_SAMPLER_DEFAULT = 0,
+    EUCHIP_SAMPLER_LINEAR = 1, EUCHIP_SAMPLER_LINEAR_CLAMPED = 2, EUCHIP_SAMPLER_LINEAR_TILED = 3,
+    EUCHIP_SAMPLER_LINEAR_MIRRORED = 4,
+    EUCHIP_SAMPLER_NEAREST = 5, EUCHIP_SAMPLER_NEAREST_CLAMPED = 6, EUCHIP_SAMPLER_NEAREST_TILED = 7,
+    EUCHIP_SAMPLER_NEAREST_MIRRORED = 8
+} euchip_sampler;
+/* Fixed geometry stages (Pipeline::geometry, src/pipeline.rs:221-229) of each
+ * triangle [a, b, c]: output it (the default); output [a, c, b]; output both,
+ * in that order (each output is its own primitive). */
+typedef enum { EUCHIP_GEOMETRY_IDENTITY = 0, EUCHIP_GEOMETRY_REVERSED = 1, EUCHIP_GEOMETRY_BOTH_SIDES = 2 } euchip_geometry;
+/* PrimitiveKind of the vertex stream (src/primitives.rs:18-104). */
+typedef enum {
+    EUCHIP_PRIMS_TRIANGLE_LIST = 0, EUCHIP_PRIMS_LINE_LIST = 1, EUCHIP_PRIMS_LINE_TRIANGLE_LIST = 2
+} euchip_primitives;
 /* Texel formats: RGBA8 packed little-endian in a uint32 ([u8;4]), or f32. */
 typedef enum { EUCHIP_FORMAT_RGBA8 = 0, EUCHIP_FORMAT_F32 = 1 } euchip_format;
 
@@ -95,6 +118,10 @@ typedef struct euchip_pipeline_desc {
     float clear_depth;
     const euchip_buffer *texture; /* RGBA8 texture for TEXTURED / TEX_SHADOW, else NULL */
     const euchip_buffer *shadow;  /* f32 shadow map for *_SHADOW, else NULL             */
+    uint32_t pixel_format; /* euchip_pixel_format                                   */
+    uint32_t sampler;      /* euchip_sampler                                        */
+    uint32_t geometry;     /* euchip_geometry (TRIANGLE_LIST streams)               */
+    uint32_t primitives;   /* euchip_primitives                                     */
 } euchip_pipeline_desc;
 
 /* Uniform block: 64 floats.

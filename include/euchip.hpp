@@ -216,6 +216,13 @@ public:
     virtual uint32_t threads() const { return 8; }
     virtual const Buffer2d<uint32_t> *texture() const { return nullptr; }
     virtual const Buffer2d<float> *shadow() const { return nullptr; }
+    // the blend's u32 layout, the texture sampler, the fixed geometry stage and
+    // the PrimitiveKind (TriangleList by default, as every reference example but
+    // wireframes.rs)
+    virtual euchip_pixel_format pixel_format() const { return EUCHIP_PIXEL_RGBA8; }
+    virtual euchip_sampler sampler() const { return EUCHIP_SAMPLER_DEFAULT; }
+    virtual euchip_geometry geometry() const { return EUCHIP_GEOMETRY_IDENTITY; }
+    virtual euchip_primitives primitives() const { return EUCHIP_PRIMS_TRIANGLE_LIST; }
 
     euchip_pipeline_desc desc() const {
         euchip_pipeline_desc d{};
@@ -240,6 +247,10 @@ public:
         d.threads = threads();
         d.texture = texture() ? texture()->handle() : nullptr;
         d.shadow = shadow() ? shadow()->handle() : nullptr;
+        d.pixel_format = pixel_format();
+        d.sampler = sampler();
+        d.geometry = geometry();
+        d.primitives = primitives();
         return d;
     }
 

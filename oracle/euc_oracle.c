@@ -215,6 +215,43 @@ void orc_sample(const orc_texture *t, float x, float y, int32_t clamped, float *
     }
 }
 
+/* f32::rem_euclid: r = x % m (exact), then r + |m| when r < 0 */
+float orc_rem_euclid(float x, float m) {
+    float r = fmodf(x, m);
+    return r < 0.0f ? r + fabsf(m) : r;
+}
+
+/* Denormalize f32 -> usize (math.rs:52-55): ((u * size as f32).max(0.0) as usize).min(size - 1) */
+static size_t denormalize(float u, size_t size) {
+    size_t v = sat_usize(rmax(u * (float)size, 0.0f));
+    return v < size - 1 ? v : size - 1;
+}
+
+/* The index map of a sampler wrapper (sampler/mod.rs:98-179): none, Clamped,
+ * Tiled, Mirrored. */
+static float wrap_index(int wrap, float x) {
+    switch (wrap) {
+    case 1: return rmin(rmax(x, 0.0f), 1.0f);
+    case 2: return orc_rem_euclid(x, 1.0f);
+    case 3: return orc_rem_euclid(x, 2.0f) >= 1.0f ? 1.0f - orc_rem_euclid(x, 1.0f) : orc_rem_euclid(x, 1.0f);
+    }
+    return x;
+}
+
+/* Sampler `kind` (ORC_SAMPLER_LINEAR .. ORC_SAMPLER_NEAREST_MIRRORED): the
+ * wrapper's index map, then Linear (sampler/linear.rs:30-66) or Nearest
+ * (sampler/nearest.rs:20-38). */
+void orc_sample_with(const orc_texture *t, int32_t kind, float u, float v, float *out) {
+    const int nearest = kind >= ORC_SAMPLER_NEAREST;
+    const int wrap = nearest ? kind - ORC_SAMPLER_NEAREST : kind - ORC_SAMPLER_LINEAR;
+    u = wrap_index(wrap, u);
+    v = wrap_index(wrap, v);
+    if (nearest)
+        tex_read(t, denormalize(u, (size_t)t->w), denormalize(v, (size_t)t->h), out);
+    else
+        orc_sample(t, u, v, 0, out);
+}
+
 static const float *vattr(const float *base, int comps, int64_t vi) { return base + (size_t)comps * (size_t)vi; }
 
 /* Pipeline::vertex for the fixed shader set. */
@@ -313,7 +350,7 @@ static void fragment_shader(const Pipe *p, const float *vd, Frag *f) {
     case ORC_SHADER_DEPTH_ONLY:
         break;
     case ORC_SHADER_TEXTURED:
-        orc_sample(p->tex, vd[0], vd[1], 0, f->c);
+        orc_sample_with(p->tex, p->d->sampler ? p->d->sampler : ORC_SAMPLER_LINEAR, vd[0], vd[1], f->c);
         break;
     case ORC_SHADER_PHONG_SHADOW: {
         /* examples/teapot.rs:99-132 with an exact x^30 chain */
@@ -341,7 +378,7 @@ static void fragment_shader(const Pipe *p, const float *vd, Frag *f) {
         const float *L = u + U_L0;
         float ndl = rmax(dot3s(n, L), 0.0f);
         float light = u[U_K + 0] + (in_light(p, vd + 8) ? ndl * u[U_K + 1] : 0.0f);
-        orc_sample(p->tex, vd[6], vd[7], 1, texc);
+        orc_sample_with(p->tex, p->d->sampler ? p->d->sampler : ORC_SAMPLER_LINEAR_CLAMPED, vd[6], vd[7], texc);
         float s = light * (1.0f / 255.0f);
         f->c[0] = texc[0] * s;
         f->c[1] = texc[1] * s;
@@ -365,6 +402,11 @@ static uint32_t blend_shader(const Pipe *p, uint32_t old, const Frag *f) {
         for (int c = 0; c < 4; c++) b[c] = sat_u8(f->c[c]);
     } else {
         for (int c = 0; c < 4; c++) b[c] = sat_u8(rmin(rmax(f->c[c], 0.0f), 1.0f) * 255.0f);
+    }
+    if (p->d->pixel_format == ORC_PIXEL_BGRA8) { /* u32::from_le_bytes([b, g, r, a]) */
+        const uint8_t r = b[0];
+        b[0] = b[2];
+        b[2] = r;
     }
     return (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
 }
@@ -418,13 +460,28 @@ typedef struct {
     int64_t frags;
 } Blitter;
 
-/* Interpolation state captured by triangles.rs:274-294's get_v_data closure. */
+/* Interpolation state captured by the get_v_data closures of triangles.rs:274-294
+ * and, with `line` set, lines.rs:99-112. */
 typedef struct {
     float o[3], dx[3], dy[3];
     const float *vout[3];
+    int line, use_x;
+    float vs0x, vs0y, norm;
 } VData;
 
 static void get_v_data(const Blitter *b, const VData *g, float x, float y, float *vd) {
+    if (g->line) {
+        /* frac = (if use_x { x - vs0.x } else { y - vs0.y }) * norm;
+         * V::weighted_sum2(v0, v1, 1.0 - frac, frac) = v0 * w0 + v1 * w1 (math.rs:34-36) */
+        float frac = (g->use_x ? x - g->vs0x : y - g->vs0y) * g->norm;
+        float w0 = 1.0f - frac;
+        if (b->pipe->frag_kind == 1) {
+            vd[0] = g->vout[0][0];
+            return;
+        }
+        for (int c = 0; c < b->pipe->nvd; c++) vd[c] = g->vout[0][c] * w0 + g->vout[1][c] * frac;
+        return;
+    }
     float t1[3], wh[3];
     float dyy[3] = {g->dy[0] * y, g->dy[1] * y, g->dy[2] * y};
     float dxx[3] = {g->dx[0] * x, g->dx[1] * x, g->dx[2] * x};
@@ -664,21 +721,133 @@ static void rasterize_rows(Blitter *b, const Coords *coords, const Setup *S, con
     }
 }
 
-static void rasterize_all(Blitter *b, const Coords *coords, int cull_mode, const SVert *verts, int64_t nverts) {
+/* `f as isize` (saturating, NaN -> 0). */
+static int64_t sat_isize(float f) {
+    if (f != f) return 0;
+    if (f >= 9223372036854775808.0f) return INT64_MAX;
+    if (f <= -9223372036854775808.0f) return INT64_MIN;
+    return (int64_t)f;
+}
+
+/* f32::clamp (NaN stays NaN). */
+static float fclamp(float x, float lo, float hi) {
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return x;
+}
+
+/* Line endpoints further than this from the origin (in pixels) are not drawn:
+ * the clipline crate's isize arithmetic is not defined there. */
+#define LINE_COORD_LIMIT ((int64_t)1 << 29)
+
+/* The Bresenham line (x1, y1) -> (x2, y2), both ends included, restricted to the
+ * rectangle [cx0, cx1] x [cy0, cy1] -- the contract of `clipline::clipline`
+ * (clipline 0.2, not vendored: parity unpinned at this boundary): clipping never
+ * moves a pixel.  Along the major axis (the longer |delta|) step k's minor
+ * coordinate is m1 + sign * floor((2k|dm| + |dM| - 1) / (2|dM|)), the line's
+ * value rounded to nearest with halves rounded back towards the start (the
+ * decision `e > 0` of an error term e = 2|dm| - |dM|).  Calls plot(x, y). */
+typedef void (*PlotFn)(void *ctx, int64_t x, int64_t y);
+
+static void clip_line(int64_t x1, int64_t y1, int64_t x2, int64_t y2, int64_t cx0, int64_t cy0, int64_t cx1,
+                      int64_t cy1, PlotFn plot, void *ctx) {
+    if (cx1 < cx0 || cy1 < cy0) return;
+    const int64_t dx = x2 - x1, dy = y2 - y1;
+    const int64_t adx = dx < 0 ? -dx : dx, ady = dy < 0 ? -dy : dy;
+    const int xmajor = adx >= ady;
+    const int64_t M1 = xmajor ? x1 : y1, m1 = xmajor ? y1 : x1;
+    const int64_t dM = xmajor ? adx : ady, dm = xmajor ? ady : adx;
+    const int64_t sM = (xmajor ? dx : dy) < 0 ? -1 : 1, sm = (xmajor ? dy : dx) < 0 ? -1 : 1;
+    const int64_t c0 = xmajor ? cx0 : cy0, c1 = xmajor ? cx1 : cy1;
+    const int64_t n0 = xmajor ? cy0 : cx0, n1 = xmajor ? cy1 : cx1;
+    /* steps k in [0, dM] whose major coordinate M1 + sM k lies in [c0, c1] */
+    int64_t klo = sM > 0 ? c0 - M1 : M1 - c1, khi = sM > 0 ? c1 - M1 : M1 - c0;
+    if (klo < 0) klo = 0;
+    if (khi > dM) khi = dM;
+    for (int64_t k = klo; k <= khi; k++) {
+        const int64_t j = dM ? (2 * k * dm + dM - 1) / (2 * dM) : 0;
+        const int64_t M = M1 + sM * k, m = m1 + sm * j;
+        if (m < n0 || m > n1) continue;
+        if (xmajor) plot(ctx, M, m);
+        else plot(ctx, m, M);
+    }
+}
+
+typedef struct {
+    Blitter *b;
+    const Coords *coords;
+    VData *g;
+    float e0z, e1z;
+} LinePlot;
+
+static void line_pixel(void *ctx, int64_t xi, int64_t yi) {
+    LinePlot *L = (LinePlot *)ctx;
+    const size_t x = (size_t)xi, y = (size_t)yi;
+    /* frac of lines.rs:88-92, z of :95 */
+    float frac = (L->g->use_x ? (float)x - L->g->vs0x : (float)y - L->g->vs0y) * L->g->norm;
+    float z = L->e0z + frac * (L->e1z - L->e0z);
+    if (passes_z_clip(L->coords, z) && test_fragment(L->b, x, y, z)) emit_fragment(L->b, x, y, L->g, z);
+}
+
+/* Lines::rasterize (lines.rs:15-121) of the pairs of the stream `vidx`. */
+static void rasterize_lines(Blitter *b, const Coords *coords, const SVert *verts, const int64_t *vidx,
+                            int64_t nverts) {
+    const float size0 = (float)b->tgt_size[0], size1 = (float)b->tgt_size[1];
+    const float flip1 = coords->y_up ? -1.0f : 1.0f;
+    const float smin0 = (float)b->tgt_min[0], smin1 = (float)b->tgt_min[1];
+    const float smax0 = (float)b->tgt_max[0], smax1 = (float)b->tgt_max[1];
+    for (int64_t l = 0; l + 1 < nverts; l += 2) {
+        b->primitive_count = b->primitive_count + 1;
+        const SVert *v[2] = {&verts[vidx[l]], &verts[vidx[l + 1]]};
+        float euc[2][3], vs[2][2];
+        for (int i = 0; i < 2; i++) {
+            const float hom[4] = {v[i]->pos[0] * 1.0f, v[i]->pos[1] * flip1, v[i]->pos[2], v[i]->pos[3]};
+            const float w = rmax(hom[3], 0.0001f);
+            euc[i][0] = hom[0] / w, euc[i][1] = hom[1] / w, euc[i][2] = hom[2] / w;
+            vs[i][0] = size0 * (euc[i][0] * 0.5f + 0.5f);
+            vs[i][1] = size1 * (euc[i][1] * -0.5f + 0.5f);
+        }
+        const int64_t x1 = sat_isize(vs[0][0]), y1 = sat_isize(vs[0][1]);
+        const int64_t x2 = sat_isize(vs[1][0]), y2 = sat_isize(vs[1][1]);
+        const int64_t wx1 = sat_isize(fclamp(rmin(vs[0][0], vs[1][0]) + 0.0f, smin0, smax0));
+        const int64_t wy1 = sat_isize(fclamp(rmin(vs[0][1], vs[1][1]) + 0.0f, smin1, smax1));
+        const int64_t wx2 = sat_isize(fclamp(rmax(vs[0][0], vs[1][0]) + 1.0f, smin0, smax0));
+        const int64_t wy2 = sat_isize(fclamp(rmax(vs[0][1], vs[1][1]) + 1.0f, smin1, smax1));
+        if (x1 <= -LINE_COORD_LIMIT || x1 >= LINE_COORD_LIMIT || y1 <= -LINE_COORD_LIMIT || y1 >= LINE_COORD_LIMIT ||
+            x2 <= -LINE_COORD_LIMIT || x2 >= LINE_COORD_LIMIT || y2 <= -LINE_COORD_LIMIT || y2 >= LINE_COORD_LIMIT)
+            continue;
+        const int64_t adx = x1 - x2 < 0 ? x2 - x1 : x1 - x2, ady = y1 - y2 < 0 ? y2 - y1 : y1 - y2;
+        VData g;
+        memset(&g, 0, sizeof(g));
+        g.line = 1;
+        g.use_x = adx > ady;
+        g.vs0x = vs[0][0], g.vs0y = vs[0][1];
+        g.norm = 1.0f / (g.use_x ? vs[1][0] - vs[0][0] : vs[1][1] - vs[0][1]);
+        g.vout[0] = v[0]->vd, g.vout[1] = v[1]->vd;
+        LinePlot L = {b, coords, &g, euc[0][2], euc[1][2]};
+        clip_line(x1, y1, x2, y2, wx1, wy1, wx2 - 1, wy2 - 1, line_pixel, &L);
+    }
+}
+
+/* verts: the shaded vertex stream; vidx: the stream after the geometry stage,
+ * as indices into verts (pipeline.rs:277-289). */
+static void rasterize_all(Blitter *b, const Coords *coords, int cull_mode, const SVert *verts, const int64_t *vidx,
+                          int64_t nverts) {
     for (int64_t t = 0; t + 2 < nverts; t += 3) {
         b->primitive_count = b->primitive_count + 1; /* begin_primitive (wrapping_add) */
         float hom_in[3][4];
-        for (int i = 0; i < 3; i++) memcpy(hom_in[i], verts[t + i].pos, sizeof(hom_in[i]));
+        for (int i = 0; i < 3; i++) memcpy(hom_in[i], verts[vidx[t + i]].pos, sizeof(hom_in[i]));
         Setup S;
         int order[3];
         if (!setup_triangle(hom_in, b->tgt_size[0], b->tgt_size[1], b->tgt_min, b->tgt_max, coords, cull_mode, &S,
                             order))
             continue;
         VData g;
+        memset(&g, 0, sizeof(g));
         memcpy(g.o, S.o, sizeof(g.o));
         memcpy(g.dx, S.dx, sizeof(g.dx));
         memcpy(g.dy, S.dy, sizeof(g.dy));
-        for (int i = 0; i < 3; i++) g.vout[i] = verts[t + order[i]].vd;
+        for (int i = 0; i < 3; i++) g.vout[i] = verts[vidx[t + order[i]]].vd;
         rasterize_rows(b, coords, &S, &g);
     }
 }
@@ -690,7 +859,8 @@ static void rasterize_all(Blitter *b, const Coords *coords, int cull_mode, const
 typedef struct {
     const Pipe *pipe;
     const SVert *verts;
-    int64_t nverts;
+    const int64_t *vidx;   /* the stream after the geometry stage (indices into verts) */
+    int64_t nverts;        /* its length */
     size_t tgt_size[2];
     uint32_t *pixel;
     size_t pw;
@@ -731,7 +901,10 @@ static void render_inner(RenderJob *j, size_t by0, size_t by1) {
         b.msaa_buf = (MsaaTexel *)malloc(sizeof(MsaaTexel) * b.msaa_w * mh);
         for (size_t i = 0; i < b.msaa_w * mh; i++) b.msaa_buf[i].prim = UINT64_MAX;
     }
-    rasterize_all(&b, &j->coords, d->cull, j->verts, j->nverts);
+    if (d->primitives == ORC_PRIMS_TRIANGLE_LIST)
+        rasterize_all(&b, &j->coords, d->cull, j->verts, j->vidx, j->nverts);
+    else
+        rasterize_lines(&b, &j->coords, j->verts, j->vidx, j->nverts);
     free(b.msaa_buf);
     atomic_fetch_add(&j->frags, b.frags);
 }
@@ -778,9 +951,13 @@ static int render_impl(const orc_desc *d, const float *u, const orc_stream *s, c
     case ORC_SHADER_TEX_SHADOW: p.nvd = 11; break;
     default: return ORC_INVALID;
     }
-    /* vertex shading of the whole stream (render_par collects first, pipeline.rs:322) */
+    if (d->primitives < ORC_PRIMS_TRIANGLE_LIST || d->primitives > ORC_PRIMS_LINE_TRIANGLE_LIST) return ORC_INVALID;
+    if (d->primitives == ORC_PRIMS_LINE_LIST && d->geometry != ORC_GEOMETRY_IDENTITY) return ORC_INVALID;
+    /* vertex shading of the whole stream (render_par collects first, pipeline.rs:322);
+     * a trailing partial primitive is dropped (primitives.rs:28-32, 96-100) */
+    const int64_t per = d->primitives == ORC_PRIMS_LINE_LIST ? 2 : 3;
     int64_t total = s->count * (int64_t)s->n_instances;
-    int64_t nverts = total - total % 3; /* TriangleList drops a trailing partial primitive */
+    int64_t nverts = total - total % per;
     SVert *verts = (SVert *)malloc(sizeof(SVert) * (size_t)(nverts > 0 ? nverts : 1));
     for (int64_t e = 0; e < nverts; e++) {
         int32_t inst = (int32_t)(e / s->count);
@@ -788,12 +965,36 @@ static int render_impl(const orc_desc *d, const float *u, const orc_stream *s, c
         int64_t vi = s->idx ? (int64_t)s->idx[k] : k;
         vertex_shader(&p, inst, vi, &verts[e]);
     }
+    /* the geometry stage (each triangle [a, b, c] becomes its outputs, in order),
+     * then PrimitiveKind::primitive_vertices: a LineTriangleList triangle is the
+     * lines a-b, b-c, c-a (primitives.rs:71-84) */
+    const int64_t k = d->geometry == ORC_GEOMETRY_BOTH_SIDES ? 2 : 1;
+    const int64_t emit = d->primitives == ORC_PRIMS_LINE_TRIANGLE_LIST ? 6 : 3;
+    int64_t *vidx = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nverts * k * 2 > 0 ? nverts * k * 2 : 1));
+    int64_t nv = 0;
+    if (d->primitives == ORC_PRIMS_LINE_LIST) {
+        for (int64_t e = 0; e < nverts; e++) vidx[nv++] = e;
+    } else {
+        for (int64_t t = 0; t + 2 < nverts; t += 3) {
+            for (int out = 0; out < 2; out++) {
+                if (out == 0 && d->geometry == ORC_GEOMETRY_REVERSED) continue;
+                if (out == 1 && d->geometry == ORC_GEOMETRY_IDENTITY) continue;
+                const int64_t a = t, b = out ? t + 2 : t + 1, c = out ? t + 1 : t + 2;
+                if (emit == 3) {
+                    vidx[nv++] = a, vidx[nv++] = b, vidx[nv++] = c;
+                } else {
+                    vidx[nv++] = a, vidx[nv++] = b, vidx[nv++] = b, vidx[nv++] = c, vidx[nv++] = c, vidx[nv++] = a;
+                }
+            }
+        }
+    }
     size_t msaa_level = d->aa_msaa ? (size_t)(d->aa_level < 0 ? 0 : (d->aa_level > 6 ? 6 : d->aa_level)) : 0;
     RenderJob j;
     memset(&j, 0, sizeof(j));
     j.pipe = &p;
     j.verts = verts;
-    j.nverts = nverts;
+    j.vidx = vidx;
+    j.nverts = nv;
     j.tgt_size[0] = tgt[0], j.tgt_size[1] = tgt[1];
     j.pixel = pixel, j.pw = psz[0];
     j.depth = depth, j.dw = dsz[0];
@@ -824,6 +1025,7 @@ static int render_impl(const orc_desc *d, const float *u, const orc_stream *s, c
         }
     }
     if (frags_out) *frags_out = atomic_load(&j.frags);
+    free(vidx);
     free(verts);
     return rc;
 }
@@ -909,4 +1111,22 @@ int orc_fragment(const orc_desc *d, const float *u, const orc_texture *tex, cons
     memcpy(out, f.c, sizeof(f.c));
     if (blended) *blended = blend_shader(&p, old_pixel, &f);
     return ORC_OK;
+}
+
+typedef struct {
+    int64_t *out;
+    int64_t n, max;
+} PlotCollect;
+
+static void collect_pixel(void *ctx, int64_t x, int64_t y) {
+    PlotCollect *c = (PlotCollect *)ctx;
+    if (c->n < c->max) c->out[2 * c->n] = x, c->out[2 * c->n + 1] = y;
+    c->n++;
+}
+
+int64_t orc_clip_line(int64_t x1, int64_t y1, int64_t x2, int64_t y2, int64_t cx0, int64_t cy0, int64_t cx1,
+                      int64_t cy1, int64_t *out_xy, int64_t max) {
+    PlotCollect c = {out_xy, 0, max};
+    clip_line(x1, y1, x2, y2, cx0, cy0, cx1, cy1, collect_pixel, &c);
+    return c.n;
 }

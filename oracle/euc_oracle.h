@@ -31,6 +31,24 @@ enum {
     ORC_SHADER_TEX_SHADOW = 7   /* texture x sun light x shadow-map lookup           */
 };
 
+/* u32 pixel layouts of the Rgba<f32> shaders' blend: [r, g, b, a] or, as the
+ * teapot example's window framebuffer (examples/teapot.rs:135-140), [b, g, r, a]
+ * little-endian. */
+enum { ORC_PIXEL_RGBA8 = 0, ORC_PIXEL_BGRA8 = 1 };
+/* Texture samplers (src/sampler/mod.rs, linear.rs, nearest.rs): filter x wrapping. */
+enum {
+    ORC_SAMPLER_DEFAULT = 0,         /* TEXTURED: Linear, TEX_SHADOW: Clamped<Linear>   */
+    ORC_SAMPLER_LINEAR = 1, ORC_SAMPLER_LINEAR_CLAMPED = 2, ORC_SAMPLER_LINEAR_TILED = 3,
+    ORC_SAMPLER_LINEAR_MIRRORED = 4,
+    ORC_SAMPLER_NEAREST = 5, ORC_SAMPLER_NEAREST_CLAMPED = 6, ORC_SAMPLER_NEAREST_TILED = 7,
+    ORC_SAMPLER_NEAREST_MIRRORED = 8
+};
+/* Fixed geometry stages (Pipeline::geometry, src/pipeline.rs:221-229) of a
+ * triangle [a, b, c]: output it; output [a, c, b]; output both, in that order. */
+enum { ORC_GEOMETRY_IDENTITY = 0, ORC_GEOMETRY_REVERSED = 1, ORC_GEOMETRY_BOTH_SIDES = 2 };
+/* PrimitiveKind (src/primitives.rs): TriangleList, LineList, LineTriangleList. */
+enum { ORC_PRIMS_TRIANGLE_LIST = 0, ORC_PRIMS_LINE_LIST = 1, ORC_PRIMS_LINE_TRIANGLE_LIST = 2 };
+
 enum { ORC_DEPTH_NONE = 0, ORC_DEPTH_LESS = 1, ORC_DEPTH_EQUAL = 2, ORC_DEPTH_GREATER = 3 };
 enum { ORC_CULL_NONE = 0, ORC_CULL_BACK = 1, ORC_CULL_FRONT = 2 };
 enum { ORC_BANDS_PAR = 0, ORC_BANDS_SEQ = 1 };
@@ -58,6 +76,10 @@ typedef struct {
     int32_t band_policy;    /* ORC_BANDS_*                                          */
     int32_t threads;        /* worker threads for PAR (available_parallelism)       */
     int32_t group_rows;     /* tests only: PAR band height override (0 = 20000*2^L/W) */
+    int32_t pixel_format;   /* ORC_PIXEL_*: how blend packs Rgba<u8> into the u32 texel  */
+    int32_t sampler;        /* ORC_SAMPLER_*: texture sampler (0 = the shader's own)  */
+    int32_t geometry;       /* ORC_GEOMETRY_*: the fixed Pipeline::geometry stage     */
+    int32_t primitives;     /* ORC_PRIMS_*: PrimitiveKind of the vertex stream        */
 } orc_desc;
 
 /* Vertex stream: element e -> (instance e / count, vertex idx[e % count]). */
@@ -105,12 +127,23 @@ int orc_setup(const float hom_in[12], int32_t w, int32_t h, int32_t by0, int32_t
               int32_t y_up, int32_t z_clip, float z_min, float z_max, int32_t cull,
               float *out);
 
+/* The pixels (x, y pairs into out_xy, at most max) of the clipped Bresenham line
+ * the lines rasterizer draws (lines.rs:83-117); returns their number. */
+int64_t orc_clip_line(int64_t x1, int64_t y1, int64_t x2, int64_t y2, int64_t cx0, int64_t cy0, int64_t cx1,
+                      int64_t cy1, int64_t *out_xy, int64_t max);
+
 /* a + d + d + ... (n times) in f32, the row walk of triangles.rs:301. */
 float orc_walk(float a, float d, int64_t n);
 
 /* Bilinear Clamped<Linear> sample of a texture (sampler/mod.rs:183-186,
  * sampler/linear.rs:30-66); out gets 4 (RGBA8 -> f32) or 1 (f32) values. */
 void orc_sample(const orc_texture *t, float u, float v, int32_t clamped, float *out);
+
+/* A sample through sampler `kind` (ORC_SAMPLER_LINEAR ... NEAREST_MIRRORED). */
+void orc_sample_with(const orc_texture *t, int32_t kind, float u, float v, float *out);
+
+/* f32::rem_euclid (the Tiled / Mirrored index maps, src/sampler/mod.rs:146-179). */
+float orc_rem_euclid(float x, float m);
 
 /* Pipeline::vertex of the fixed shader set for stream vertex `vi` of instance
  * `inst`: the clip position and the VertexData (returns its float count). */

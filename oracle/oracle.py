@@ -34,6 +34,7 @@ class OrcDesc(C.Structure):
         ("z_clip", C.c_int32), ("z_min", C.c_float), ("z_max", C.c_float),
         ("cull", C.c_int32), ("aa_msaa", C.c_int32), ("aa_level", C.c_int32),
         ("band_policy", C.c_int32), ("threads", C.c_int32), ("group_rows", C.c_int32),
+        ("pixel_format", C.c_int32), ("sampler", C.c_int32), ("geometry", C.c_int32), ("primitives", C.c_int32),
     ]
 
 
@@ -81,6 +82,12 @@ def lib():
         L.orc_fragment.restype = C.c_int
         L.orc_fragment.argtypes = [C.POINTER(OrcDesc), C.c_void_p, C.POINTER(OrcTexture), C.POINTER(OrcTexture),
                                    C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]
+        L.orc_sample_with.restype = None
+        L.orc_sample_with.argtypes = [C.POINTER(OrcTexture), C.c_int32, C.c_float, C.c_float, C.c_void_p]
+        L.orc_rem_euclid.restype = C.c_float
+        L.orc_rem_euclid.argtypes = [C.c_float, C.c_float]
+        L.orc_clip_line.restype = C.c_int64
+        L.orc_clip_line.argtypes = [C.c_int64] * 8 + [C.c_void_p, C.c_int64]
         L.orc_lighting2.restype = None
         L.orc_lighting2.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -98,7 +105,9 @@ def make_desc(p) -> OrcDesc:
         depth_write=int(p.depth_write), y_up=int(p.y_up), z_clip=int(p.z_clip),
         z_min=float(p.z_min), z_max=float(p.z_max), cull=p.cull, aa_msaa=int(p.aa_msaa),
         aa_level=int(p.aa_level), band_policy=p.band_policy, threads=int(p.threads),
-        group_rows=int(getattr(p, "band_rows_override", 0)))
+        group_rows=int(getattr(p, "band_rows_override", 0)), pixel_format=int(getattr(p, "pixel_format", 0)),
+        sampler=int(getattr(p, "sampler", 0)), geometry=int(getattr(p, "geometry", 0)),
+        primitives=int(getattr(p, "primitives", 0)))
 
 
 def _texture(t):
@@ -190,6 +199,17 @@ def sample(texture, u, v, clamped=True):
     return out[:1] if tex.is_f32 else out
 
 
+def sample_with(texture, kind, u, v):
+    tex, arr = _texture(texture)
+    out = np.zeros(4, dtype=np.float32)
+    lib().orc_sample_with(C.byref(tex), int(kind), float(u), float(v), out.ctypes.data)
+    return out[:1] if tex.is_f32 else out
+
+
+def rem_euclid(x, m):
+    return np.float32(lib().orc_rem_euclid(float(x), float(m)))
+
+
 def lighting2(uniforms, wpos, nrm):
     u = np.ascontiguousarray(uniforms, dtype=np.float32)
     a = np.ascontiguousarray(wpos, dtype=np.float32)
@@ -235,3 +255,12 @@ def fragment(desc, uniforms, vd, texture=None, shadow=None, old_pixel=0):
     assert rc == 0
     del ta, sa
     return out, blended.value
+
+
+def clip_line(p1, p2, rect_min, rect_max):
+    """Pixels [(x, y), ...] of the clipped Bresenham line of the lines rasterizer."""
+    out = np.zeros(2 * 8192, np.int64)
+    n = lib().orc_clip_line(int(p1[0]), int(p1[1]), int(p2[0]), int(p2[1]), int(rect_min[0]), int(rect_min[1]),
+                            int(rect_max[0]), int(rect_max[1]), out.ctypes.data, 8192)
+    assert n <= 8192
+    return [tuple(map(int, out[2 * i:2 * i + 2])) for i in range(n)]
