@@ -104,6 +104,8 @@ SIGNATURES = {
                                 C.c_void_p]),
     "euchip_render_async": (C.c_int, [C.c_void_p, C.POINTER(PipelineDescC), C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
+    "euchip_render_noalloc": (C.c_int, [C.c_void_p, C.POINTER(PipelineDescC), C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p]),
     "euchip_ctx_bytes_in_use": (C.c_size_t, [C.c_void_p]),
     "euchip_ctx_set_budget": (C.c_int, [C.c_void_p, C.c_size_t]),
     "euchip_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),

@@ -191,6 +191,31 @@ struct FrameParams {
     View v0;
 };
 
+// euchip_render_noalloc: every per-frame buffer is carved from one static device
+// array (noalloc.hip) of kNoAllocBytes, laid out below; requests beyond the
+// limits are rejected.
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+constexpr uint32_t kNaMaxTris = 1u << 18;         // triangles (instances x count / 3)
+constexpr uint32_t kNaMaxVaryFloats = 3u << 20;   // shaded vertices x VertexData floats
+constexpr uint32_t kNaMaxTiles = 1u << 14;        // 32 x 8 tiles, e.g. 2048 x 2048 pixels
+constexpr uint32_t kNaBinSlots = 3u << 20;        // bin slots over all tiles
+constexpr size_t kNaRec0 = 0;
+constexpr size_t kNaRec1 = kNaRec0 + (size_t)kNaMaxTris * kRec0Quads * 16;
+constexpr size_t kNaVary = kNaRec1 + (size_t)kNaMaxTris * kRec1Quads * 16;
+constexpr size_t kNaBins = kNaVary + (size_t)kNaMaxVaryFloats * 4;
+constexpr size_t kNaTiles = kNaBins + (size_t)kNaBinSlots * 4;
+constexpr size_t kNaBusy = kNaTiles + (size_t)kNaMaxTiles * sizeof(TileBin);
+constexpr size_t kNaHeavy = kNaBusy + (size_t)8 * kNaMaxTiles * 4;
+constexpr size_t kNaBig = kNaHeavy + (size_t)kNaMaxTiles * 4;
+constexpr size_t kNaCtr = kNaBig + (size_t)kBigCap * 4;
+constexpr size_t kNaPool = kNaCtr + ((sizeof(Counters) + 255) / 256) * 256;
+constexpr size_t kNaPoolWords = (kNoAllocBytes - kNaPool) / 4;
+static_assert(kNaPool < kNoAllocBytes && kNaPoolWords >= (1u << 20), "no-allocation scratch layout");
+
+// The device address of the no-allocation scratch array on `device` (the
+// current device), looked up once per device (noalloc.hip).
+hipError_t noalloc_scratch(int device, void **out);
+
 // Kernel launchers (kernels.hip).  All enqueue on `s`.  `batched` selects the
 // kernels that read their requests from p.views (setup then runs `blocks`
 // workgroups as p.blk maps them); one-request launches read p.v0.

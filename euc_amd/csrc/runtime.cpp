@@ -84,6 +84,9 @@ struct euchip_ctx {
     uint32_t seen_dropped = 0;  // frames reported dropped before the last synchronize
     size_t bytes_in_use = 0;    // device bytes held through this context
     size_t budget = 0;          // 0: no limit
+    void *noalloc = nullptr;    // device address of the no-allocation scratch on this device
+    hipError_t noalloc_err = hipSuccess;
+    uint32_t *na_readback = nullptr;  // pinned host words: flags and pool use of a no-allocation frame
 };
 
 struct euchip_buffer {
@@ -942,7 +945,13 @@ euchip_status euchip_ctx_create(int device, euchip_ctx **out) {
     }
     memset(ctx->report, 0, sizeof(Report));
     ctx->report->frame = ~0u;  // no frame reported yet
-
+    if (hipHostMalloc((void **)&ctx->na_readback, 256, hipHostMallocDefault) != hipSuccess) {
+        euchip_ctx_destroy(ctx);
+        return EUCHIP_HIP_ERROR;
+    }
+    // the no-allocation scratch of this device (current above); a failed lookup
+    // only disables euchip_render_noalloc
+    ctx->noalloc_err = noalloc_scratch(device, &ctx->noalloc);
     *out = ctx;
     return EUCHIP_OK;
 }
@@ -971,6 +980,7 @@ euchip_status euchip_ctx_destroy(euchip_ctx *ctx) {
     reap_deferred(ctx, true);
     (void)hipFree(ctx->counters);
     if (ctx->report) (void)hipHostFree(ctx->report);
+    if (ctx->na_readback) (void)hipHostFree(ctx->na_readback);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return EUCHIP_OK;
@@ -1168,6 +1178,85 @@ euchip_status euchip_render_async(euchip_ctx *ctx, const euchip_pipeline_desc *d
                                   const euchip_mesh *mesh, euchip_buffer *pixel, euchip_buffer *depth) {
     if (ctx) (void)hipSetDevice(ctx->device);
     return render_impl(ctx, desc, uniforms, mesh, pixel, depth, false);
+}
+
+euchip_status euchip_render_noalloc(euchip_ctx *ctx, const euchip_pipeline_desc *d, const float *uniforms,
+                                    const euchip_mesh *mesh, euchip_buffer *pixel, euchip_buffer *depth) {
+    if (!ctx || !d || !uniforms || !mesh) return fail(ctx, EUCHIP_INVALID_ARG, "null argument");
+    (void)hipSetDevice(ctx->device);
+    if (!ctx->noalloc)
+        return fail(ctx, EUCHIP_HIP_ERROR, "no-allocation scratch unavailable: %s", hipGetErrorString(ctx->noalloc_err));
+    euchip_status s = check_desc(ctx, d);
+    if (!s) s = check_triangles(ctx, d);
+    if (s) return s;
+    Plan P;
+    if ((s = plan_request(ctx, d, uniforms, mesh, pixel, depth, d->texture, d->shadow, P))) return s;
+    FrameParams p;
+    memset(&p, 0, sizeof(p));
+    size_t ntiles = 0;
+    if (P.render) {
+        fill_pipe(p, d);
+        p.tile_sx = 5;
+        p.v0 = P.v;
+        tile_grid(p.v0, p.tile_sx);
+        ntiles = (size_t)p.v0.ntx * p.v0.nty;
+        // the limits of the scratch layout, checked before anything is enqueued
+        if (P.n_tris > kNaMaxTris)
+            return fail(ctx, EUCHIP_OVER_BUDGET, "no-allocation render: %llu triangles (at most %u)",
+                        (unsigned long long)P.n_tris, kNaMaxTris);
+        if (P.n_sv * vd_count(d->shader) > kNaMaxVaryFloats)
+            return fail(ctx, EUCHIP_OVER_BUDGET, "no-allocation render: %llu shaded vertices of %u floats (at most %u floats)",
+                        (unsigned long long)P.n_sv, vd_count(d->shader), kNaMaxVaryFloats);
+        if (ntiles > kNaMaxTiles)
+            return fail(ctx, EUCHIP_OVER_BUDGET, "no-allocation render: %zu tiles of 32 x 8 pixels (at most %u)", ntiles,
+                        kNaMaxTiles);
+    }
+    const hipStream_t st = ctx->stream;
+    if (P.clear_pixel)
+        HIP_TRY(ctx, launch_fill_u32((uint32_t *)pixel->ptr, d->clear_color, (size_t)pixel->w * pixel->h, st));
+    if (P.clear_depth)
+        HIP_TRY(ctx, launch_fill_u32((uint32_t *)depth->ptr, f32_bits(d->clear_depth), (size_t)depth->w * depth->h, st));
+    if (!P.render) {
+        HIP_TRY(ctx, hipStreamSynchronize(st));
+        return P.status;
+    }
+    uint8_t *base = static_cast<uint8_t *>(ctx->noalloc);
+    p.n_views = 1;
+    p.ntiles = (uint32_t)ntiles;
+    p.rec0 = reinterpret_cast<uint4 *>(base + kNaRec0);
+    p.rec1 = reinterpret_cast<uint4 *>(base + kNaRec1);
+    p.vary = reinterpret_cast<float *>(base + kNaVary);
+    p.ctr = reinterpret_cast<Counters *>(base + kNaCtr);
+    p.tile_count = reinterpret_cast<TileBin *>(base + kNaTiles);
+    p.count_next = nullptr;
+    p.zero_n = 0;
+    p.busy = reinterpret_cast<uint32_t *>(base + kNaBusy);
+    p.busy_map = BUSY_BLOCKS;
+    p.heavy = reinterpret_cast<uint32_t *>(base + kNaHeavy);
+    p.heavy_cap = kNaMaxTiles;
+    p.big = reinterpret_cast<uint32_t *>(base + kNaBig);
+    p.big_cap = P.n_tris <= ntiles ? kBigCap : 0u;
+    p.bins = reinterpret_cast<uint32_t *>(base + kNaBins);
+    p.v0.tile_cap = (uint32_t)std::min<size_t>(kNaBinSlots / ntiles, (size_t)1 << 24);
+    p.v0.bin_base = 0;
+    p.pool = reinterpret_cast<uint32_t *>(base + kNaPool);
+    p.pool_cap = (uint32_t)kNaPoolWords;
+    p.report = nullptr;
+    p.frame = 0;
+    // the array keeps the last call's contents: counters and tile bins start at
+    // zero (fill kernels: no memset nodes)
+    HIP_TRY(ctx, launch_fill_u32(reinterpret_cast<uint32_t *>(p.ctr), 0u, sizeof(Counters) / 4, st));
+    HIP_TRY(ctx, launch_fill_u32(reinterpret_cast<uint32_t *>(p.tile_count), 0u, 2 * ntiles, st));
+    HIP_TRY(ctx, launch_setup(p, st, false, 0));
+    HIP_TRY(ctx, launch_raster(p, st, false, ctx->max_groups));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->na_readback, &p.ctr->flags[0][0], 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->na_readback + 1, &p.ctr->pool[0][0], 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    if (ctx->na_readback[0] & FLAG_BIN_OVERFLOW)
+        return fail(ctx, EUCHIP_OVER_BUDGET,
+                    "no-allocation render: the frame's bin overflow needs %u words, the scratch holds %zu; its "
+                    "targets were not written", ctx->na_readback[1], kNaPoolWords);
+    return EUCHIP_OK;
 }
 
 euchip_status euchip_batch_create(euchip_ctx *ctx, euchip_batch **out) {

@@ -198,6 +198,20 @@ euchip_status euchip_render(euchip_ctx *ctx, const euchip_pipeline_desc *desc, c
 euchip_status euchip_render_async(euchip_ctx *ctx, const euchip_pipeline_desc *desc, const float *uniforms,
                                   const euchip_mesh *mesh, euchip_buffer *pixel, euchip_buffer *depth);
 
+/* Same as euchip_render (blocking, same results), but allocates no device memory:
+ * every per-frame buffer comes from one array the library holds statically on
+ * each device it runs on (64 MiB, held until the process ends).  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  Limits: at most 262,144
+ * triangles, 3 Mi VertexData floats (shaded vertices x the shader's floats),
+ * 16,384 tiles of 32 x 8 pixels (e.g. 2048 x 2048) -- beyond them it returns
+ * EUCHIP_OVER_BUDGET before enqueuing anything; a frame whose bin overflow does
+ * not fit the array's pool (about 4 Mi entries) also returns EUCHIP_OVER_BUDGET,
+ * its targets unwritten.  Every call on a device shares that device's array:
+ * calls that overlap on one device (other host threads, other contexts) race. */
+euchip_status euchip_render_noalloc(euchip_ctx *ctx, const euchip_pipeline_desc *desc, const float *uniforms,
+                                    const euchip_mesh *mesh, euchip_buffer *pixel, euchip_buffer *depth);
+
 /* ---- device memory budget ---------------------------------------------------------- */
 /* Bytes of device memory the context holds (buffers, meshes, per-frame and batch
  * scratch).  With a budget set (0 = none), an allocation that would take the total
