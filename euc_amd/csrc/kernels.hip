@@ -725,7 +725,7 @@ EUCHIP_DEV void v_data(const TriRec &R, const float (*V)[VDN<SH>::N], float X, f
 }
 
 // Deferred shading of one pixel's winning fragment (pipeline.rs:539-576).
-template <uint32_t SH>
+template <uint32_t SH, bool GS>
 EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const View &v, const TriRec &R, uint32_t px, uint32_t py,
                                 uint32_t by0, uint32_t old) {
     constexpr uint32_t NV = VDN<SH>::N;
@@ -740,7 +740,7 @@ EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const View &v, const TriRe
     const uint32_t L = p.msaa_level;
     if (L == 0u) {
         v_data<SH>(R, V, (float)px, (float)py, vd);
-        fragment_shader<SH>(v, p.sampler, vd, frag);
+        fragment_shader<SH, GS>(v, p.sampler, vd, frag);
     } else {
         // coarse shading on the band-relative 2^L grid with bilinear reconstruction
         // (pipeline.rs:541-570); tgt_min = (0, by0)
@@ -754,7 +754,7 @@ EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const View &v, const TriRe
         for (int q = 0; q < 4; q++) {
             const uint32_t ix = cx + (uint32_t)(q & 1), iy = cy + (uint32_t)(q >> 1);
             v_data<SH>(R, V, (float)(ix << L), (float)(by0 + (iy << L)), vd);
-            fragment_shader<SH>(v, p.sampler, vd, t[q]);
+            fragment_shader<SH, GS>(v, p.sampler, vd, t[q]);
         }
         if constexpr (SH == FLAT) {
             frag[0] = t[0][0];
@@ -937,7 +937,7 @@ constexpr uint32_t raster_waves(uint32_t sh) {
     return (sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 5u : sh == TEXTURED ? 7u : (sh == DEPTH_ONLY || sh == FLAT) ? 8u : 7u;
 }
 
-template <uint32_t SH, bool BATCH, bool BIG>
+template <uint32_t SH, bool BATCH, bool BIG, bool GS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(SH)))) void k_raster(FrameParams p) {
     __shared__ TileShared sh;
     const uint32_t tid = threadIdx.x;
@@ -1021,38 +1021,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         const uint32_t cap = v.tile_cap;
         const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
         __syncthreads();
-        // this tile's bin, then the big triangles (staging drops those that miss
-        // it), then (only when it overflowed) its overflow segments
+        // Runs of entries, kBatch at a time: this tile's bin and the big
+        // triangles (staging drops those that miss it), then (only when it
+        // overflowed) its overflow segments.  Segments were taken from the pool
+        // in this frame, each pointing at the one taken before it; a walk of at
+        // most pool_used / 3 segments whose every read stays below pool_used
+        // ends even on a damaged list.  One loop, so one copy of raster_batch.
         const uint32_t nb = cnt < cap ? cnt : cap;
-        const uint32_t nscan = nb + nbig;
-        for (uint32_t b = 0; b < nscan; b += kBatch) {
+        uint32_t run = nb + nbig;           // entries of the current run
+        uint32_t base = 0u;                 // pool index of the current segment's entries (0: bin + big)
+        uint32_t seg = cnt > cap ? head : 0u, guard = 0u;
+        for (uint32_t b = 0;; b += kBatch) {
+            while (b >= run) {              // next segment (uniform)
+                if (!seg || guard++ > pool_used / 3u || seg + 1u > pool_used) {
+                    seg = 0u;
+                    break;
+                }
+                const uint32_t s0 = seg - 1u;
+                seg = __builtin_amdgcn_readfirstlane(p.pool[s0]);
+                const uint32_t len = __builtin_amdgcn_readfirstlane(p.pool[s0 + 1u]);
+                base = s0 + 2u;
+                run = len < pool_used - base ? len : pool_used - base;
+                b = 0u;
+            }
+            if (b >= run) break;
             const uint32_t i = b + tid;
             bool valid = false;
             uint32_t prim = 0;
-            if (tid >= kBatch || i >= nscan) {
+            if (tid >= kBatch || i >= run) {
+            } else if (base) {
+                valid = true, prim = p.pool[base + i];
             } else if (i < nb) {
                 valid = true, prim = bin[i];
             } else {
                 valid = BIG, prim = BIG ? p.big[i - nb] : 0u;
             }
             raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, prim, g, mode);
-        }
-        // Segments were taken from the pool in this frame, each pointing at the
-        // one taken before it; a walk of at most pool_used / 3 steps whose every
-        // read stays below pool_used ends even on a damaged list.
-        uint32_t seg = cnt > cap ? head : 0u;
-        for (uint32_t guard = 0; seg && guard <= pool_used / 3u; guard++) {
-            const uint32_t s0 = seg - 1u;
-            if (s0 + 2u > pool_used) break;
-            const uint32_t nxt = __builtin_amdgcn_readfirstlane(p.pool[s0]);
-            uint32_t len = __builtin_amdgcn_readfirstlane(p.pool[s0 + 1u]);
-            len = len < pool_used - (s0 + 2u) ? len : pool_used - (s0 + 2u);
-            for (uint32_t b = 0; b < len; b += kBatch) {
-                const uint32_t i = b + tid;
-                const bool valid = tid < kBatch && i < len;
-                raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, valid ? p.pool[s0 + 2u + i] : 0u, g, mode);
-            }
-            seg = nxt;
         }
         uint32_t claimed = 0u;
         if (claim && tid == 0u) claimed = per_xcd + atomicAdd(next, 1u);
@@ -1076,7 +1080,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
                     TriRec R;
                     load_rec0(p, v.tri_base + prim, R);
                     tri_sids(v, p.geometry, prim, R.flags, R.sid);
-                    store_stream(&v.pixel[pidx], shade_pixel<SH>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
+                    store_stream(&v.pixel[pidx], shade_pixel<SH, GS>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
                                                     v.clear_color_en ? v.clear_color : 0u));
                 } else if (v.clear_color_en) {
                     store_stream(&v.pixel[pidx], v.clear_color);
@@ -1200,31 +1204,46 @@ hipError_t launch_setup(const FrameParams &p, hipStream_t s, bool batched, uint3
 // Workgroups of a raster kernel that fit on the device at once (`cus` compute
 // units); the busy-tile loop gives every workgroup a first tile, so workgroups
 // that could only start when others finish would hold theirs back.
-template <uint32_t SH, bool BATCH, bool BIG>
+template <uint32_t SH, bool BATCH, bool BIG, bool GS>
 static uint32_t resident_groups(uint32_t cus) {
     static int per_cu = 0;
     if (per_cu == 0) {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_raster<SH, BATCH, BIG>), 256, 0) !=
-                hipSuccess || n < 1)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_raster<SH, BATCH, BIG, GS>),
+                                                         256, 0) != hipSuccess ||
+            n < 1)
             n = 8;
         per_cu = n;
     }
     return (uint32_t)per_cu * cus;
 }
 
+template <uint32_t SH, bool GS>
+static void launch_raster_g(const FrameParams &p, hipStream_t s, bool batched, uint32_t groups, uint32_t cus) {
+    if (batched) {
+        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, true, false, GS>(cus) / 8u * 8u));
+        hipLaunchKernelGGL((k_raster<SH, true, false, GS>), dim3(groups), dim3(256), 0, s, p);
+    } else if (p.big_cap) {
+        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, false, true, GS>(cus) / 8u * 8u));
+        hipLaunchKernelGGL((k_raster<SH, false, true, GS>), dim3(groups), dim3(256), 0, s, p);
+    } else {
+        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, false, false, GS>(cus) / 8u * 8u));
+        hipLaunchKernelGGL((k_raster<SH, false, false, GS>), dim3(groups), dim3(256), 0, s, p);
+    }
+}
+
+// Textured shaders whose sampler is not their own take the kernels that read
+// textures through any sampler kind.
 template <uint32_t SH>
 static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, uint32_t groups, uint32_t cus) {
-    if (batched) {
-        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, true, false>(cus) / 8u * 8u));
-        hipLaunchKernelGGL((k_raster<SH, true, false>), dim3(groups), dim3(256), 0, s, p);
-    } else if (p.big_cap) {
-        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, false, true>(cus) / 8u * 8u));
-        hipLaunchKernelGGL((k_raster<SH, false, true>), dim3(groups), dim3(256), 0, s, p);
-    } else {
-        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, false, false>(cus) / 8u * 8u));
-        hipLaunchKernelGGL((k_raster<SH, false, false>), dim3(groups), dim3(256), 0, s, p);
+    if constexpr (SH == TEXTURED || SH == TEX_SHADOW) {
+        const uint32_t own = SH == TEX_SHADOW ? (uint32_t)SAMPLER_LINEAR_CLAMPED : (uint32_t)SAMPLER_LINEAR;
+        if (p.sampler != own) {
+            launch_raster_g<SH, true>(p, s, batched, groups, cus);
+            return;
+        }
     }
+    launch_raster_g<SH, false>(p, s, batched, groups, cus);
 }
 
 // Workgroups of the tile loop: a multiple of the 8 XCDs, no more than the
@@ -1467,7 +1486,7 @@ __global__ __launch_bounds__(256) void k_line_resolve(FrameParams p) {
             const uint32_t L = p.msaa_level;
             if (L == 0u) {
                 v_data_line<SH>(V, use_x, vs0x, vs0y, norm, (float)px, (float)py, vd);
-                fragment_shader<SH>(v, p.sampler, vd, frag);
+                fragment_shader<SH, true>(v, p.sampler, vd, frag);
             } else {
                 uint32_t by0, by1;
                 band_of(p, v, py, by0, by1);
@@ -1481,7 +1500,7 @@ __global__ __launch_bounds__(256) void k_line_resolve(FrameParams p) {
                 for (int q = 0; q < 4; q++) {
                     const uint32_t ix = cx + (uint32_t)(q & 1), iy = cy + (uint32_t)(q >> 1);
                     v_data_line<SH>(V, use_x, vs0x, vs0y, norm, (float)(ix << L), (float)(by0 + (iy << L)), vd);
-                    fragment_shader<SH>(v, p.sampler, vd, t[q]);
+                    fragment_shader<SH, true>(v, p.sampler, vd, t[q]);
                 }
                 if constexpr (SH == FLAT) {
                     frag[0] = t[0][0];

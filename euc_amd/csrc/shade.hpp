@@ -127,10 +127,13 @@ EUCHIP_DEV void sample_bilinear(const void *data, uint32_t w, uint32_t h, float 
     }
 }
 
-// f32::rem_euclid: x % m (exact), plus |m| when negative.
+// f32::rem_euclid for m = 1 or 2: x % m (exact), plus m when negative.  x % m =
+// x - m * trunc(x / m) with the sign of x: x / m and m * trunc(...) are exact
+// for these m, and the difference is the low bits of x, so exact too (NaN and
+// +-Inf give NaN as fmod does).
 EUCHIP_DEV float rem_euclid(float x, float m) {
-    const float r = fmodf(x, m);
-    return r < 0.0f ? r + fabsf(m) : r;
+    const float r = copysignf(x - m * truncf(x / m), x);
+    return r < 0.0f ? r + m : r;
 }
 
 // Texture sample through sampler `kind` (EUCHIP_SAMPLER_LINEAR ..
@@ -232,7 +235,10 @@ EUCHIP_DEV bool in_light(const View &v, const float lvp[3]) {
 }
 
 // Pipeline::fragment of the fixed shader set.  out: Rgba<f32> (FLAT: colour bits in out[0]).
-template <uint32_t SH>
+// GS: the texture is read through `sampler` (any EUCHIP_SAMPLER_* kind);
+// otherwise through the shader's own one (TEXTURED: Linear, TEX_SHADOW:
+// Clamped<Linear>), whose kernels then carry no code of the others.
+template <uint32_t SH, bool GS = false>
 EUCHIP_DEV void fragment_shader(const View &v, uint32_t sampler, const float *vd, float out[4]) {
     const float *u = v.u;
     out[0] = out[1] = out[2] = out[3] = 0.0f;
@@ -243,7 +249,8 @@ EUCHIP_DEV void fragment_shader(const View &v, uint32_t sampler, const float *vd
     } else if constexpr (SH == PHONG) {
         lighting2(u, vd, vd + 3, out);
     } else if constexpr (SH == TEXTURED) {
-        sample_rgba8(v.tex, v.tex_w, v.tex_h, sampler, vd[0], vd[1], out);
+        if constexpr (GS) sample_rgba8(v.tex, v.tex_w, v.tex_h, sampler, vd[0], vd[1], out);
+        else sample_bilinear<4>(v.tex, v.tex_w, v.tex_h, vd[0], vd[1], false, out);
     } else if constexpr (SH == PHONG_SHADOW) {
         float wn[3], cd[3], ld[3], t3[3];
         normalize3(vd + 3, wn);
@@ -264,7 +271,8 @@ EUCHIP_DEV void fragment_shader(const View &v, uint32_t sampler, const float *vd
         normalize3(vd + 3, n);
         float ndl = rmax(dot3s(n, u + U_L0), 0.0f);
         float light = u[U_K + 0] + (in_light(v, vd + 8) ? ndl * u[U_K + 1] : 0.0f);
-        sample_rgba8(v.tex, v.tex_w, v.tex_h, sampler, vd[6], vd[7], texc);
+        if constexpr (GS) sample_rgba8(v.tex, v.tex_w, v.tex_h, sampler, vd[6], vd[7], texc);
+        else sample_bilinear<4>(v.tex, v.tex_w, v.tex_h, vd[6], vd[7], true, texc);
         float s = light * (1.0f / 255.0f);
         out[0] = texc[0] * s;
         out[1] = texc[1] * s;

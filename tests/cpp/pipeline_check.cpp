@@ -83,8 +83,15 @@ public:
     euc::BandPolicy band_policy() const override {
         return kv_.at("bands") == "seq" ? euc::BandPolicy::Seq : euc::BandPolicy::Par;
     }
+    euchip_pixel_format pixel_format() const override { return (euchip_pixel_format)opt("pixel_format"); }
+    euchip_geometry geometry() const override { return (euchip_geometry)opt("geometry"); }
+    euchip_primitives primitives() const override { return (euchip_primitives)opt("prims"); }
 
 private:
+    unsigned long opt(const char *key) const {
+        const auto it = kv_.find(key);
+        return it == kv_.end() ? 0ul : std::stoul(it->second);
+    }
     std::map<std::string, std::string> kv_;
     euc::Uniforms u_;
 };

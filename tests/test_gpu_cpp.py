@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 from euc_amd import scenes as S
-from euc_amd.pipeline import AaMode, BandPolicy, CoordinateMode, CullMode, DepthMode, PipelineDesc, PixelMode, Shader, Xform
+from euc_amd.pipeline import (AaMode, BandPolicy, CoordinateMode, CullMode, DepthMode, Geometry, PipelineDesc,
+                              PixelFormat, PixelMode, Primitives, Shader, Xform)
 from oracle import oracle as O
 
 from parity import assert_same
@@ -46,11 +47,19 @@ CASES = [
     ("pixel_only_empty_depth", Shader.COLOR, "none", "directx", "back", -1, "seq", True, (64, 48), (0, 0), 1, 0),
     ("size_mismatch_panics", Shader.FLAT, "less_write", "vulkan", "back", -1, "seq", True, (64, 48), (32, 48), 1, 1),
     ("band_zero_panics", Shader.FLAT, "less_write", "vulkan", "back", -1, "par", True, (30000, 2), (30000, 2), 1, 1),
+    # LineList, and a LineTriangleList through the both-sides geometry stage with the BGRA layout
+    ("line_list", Shader.COLOR, "less_write", "vulkan", "none", -1, "seq", True, (90, 70), (90, 70), 1, 1,
+     {"prims": Primitives.LINE_LIST}),
+    ("line_triangles_bgra", Shader.COLOR, "none", "opengl", "back", 1, "seq", True, (90, 70), (90, 70), 1, 1,
+     {"prims": Primitives.LINE_TRIANGLE_LIST, "geometry": Geometry.BOTH_SIDES, "pixel_format": PixelFormat.BGRA8}),
+    ("triangles_reversed_bgra", Shader.COLOR, "less_write", "vulkan", "back", -1, "seq", True, (90, 70), (90, 70),
+     1, 1, {"geometry": Geometry.REVERSED, "pixel_format": PixelFormat.BGRA8}),
 ]
 
 
 def write_case(d, seed, case):
-    name, sh, dm, cm, cull, msaa, bands, pw_, (pw, ph), (dw, dh), has_p, has_d = case
+    name, sh, dm, cm, cull, msaa, bands, pw_, (pw, ph), (dw, dh), has_p, has_d = case[:12]
+    extra = case[12] if len(case) > 12 else {}
     pos, col, rgba8 = random_clip_tris(seed, 40)
     os.makedirs(d, exist_ok=True)
     pos.tofile(os.path.join(d, "pos.f32"))
@@ -59,13 +68,17 @@ def write_case(d, seed, case):
     S.uniforms().astype(np.float32).tofile(os.path.join(d, "uniforms.f32"))
     kv = {"shader": int(sh), "clip": 1, "depth_mode": dm, "pixel_write": int(pw_), "coords": cm, "cull": cull,
           "msaa": msaa, "bands": bands, "pos_comps": 4, "pw": pw, "ph": ph, "dw": dw, "dh": dh,
-          "pixel": has_p, "depth": has_d, "clear_color": "0x11223344", "clear_depth": "0.75"}
+          "pixel": has_p, "depth": has_d, "clear_color": "0x11223344", "clear_depth": "0.75",
+          **{k: int(v) for k, v in extra.items()}}
     with open(os.path.join(d, "case.txt"), "w") as f:
         for k, v in kv.items():
             f.write(f"{k} {v}\n")
     aa = AaMode.NONE if msaa < 0 else AaMode.Msaa(msaa)
     desc = PipelineDesc.make(sh, Xform.CLIP, PixelMode.WRITE if pw_ else PixelMode.PASS, DEPTH[dm], COORDS[cm],
-                             CULL[cull], aa, BandPolicy.SEQ if bands == "seq" else BandPolicy.PAR, 8)
+                             CULL[cull], aa, BandPolicy.SEQ if bands == "seq" else BandPolicy.PAR, 8,
+                             pixel_format=extra.get("pixel_format", PixelFormat.RGBA8),
+                             geometry=extra.get("geometry", Geometry.IDENTITY),
+                             primitives=extra.get("prims", Primitives.TRIANGLE_LIST))
     mesh = S.Mesh(pos=pos, count=pos.shape[0], col=col, rgba8=rgba8)
     return desc, mesh
 
