@@ -61,6 +61,7 @@ struct Counters {
     uint32_t busy[2][8][32];  // per XCD: entries of the busy-tile list
     uint32_t flags[2][32];    // FrameFlags of the frame (the pool ran out: it was dropped)
     uint32_t max_count[2][32];// largest bin count of a tile past its capacity (one-request frames)
+    unsigned long long ck[2][16];  // 16-byte slots of the checkpoint pool taken
 };
 
 // What a one-request frame's raster pass reports to host-visible memory when
@@ -71,6 +72,7 @@ struct Report {
     uint32_t flags;      // FrameFlags
     uint32_t pool_used;  // pool words the frame asked for (more than pool_cap when it ran out)
     uint32_t dropped;    // frames dropped so far on this context (running count)
+    uint32_t ck_used;    // checkpoint slots the frame asked for (saturated)
     uint32_t frame;      // the frame these counts belong to
 };
 
@@ -102,6 +104,21 @@ enum BusyMap : uint32_t { BUSY_RANGES = 0, BUSY_BLOCKS = 1 };
 // workgroup's LDS grouping overflows.
 constexpr uint32_t kBigTiles = 64;
 constexpr uint32_t kBigCap = 4096;
+
+// Row checkpoints of big triangles.  The exact walk of a row (w += dx from the
+// row start x0, triangles.rs:257-301) reaches a tile only after every pixel
+// left of it; a pass between setup and raster walks each row of each big
+// triangle once and keeps, per tile column the row's span touches, the weights
+// at the first pixel of the span in that column.  A big triangle with
+// `rows` rows and `cols` tile columns in its box takes rows * (cols + 1)
+// 16-byte slots of the checkpoint pool, column-major (slot k * rows + r, so
+// the lanes of a wave, on consecutive rows, touch consecutive slots): k = 0
+// holds (x0, x1) of row r's span, k = 1 + c the weights (w0, w1, w2) at
+// max(x0, first pixel of box column c).
+// A triangle the pool cannot hold keeps the walk from the row start.
+constexpr uint32_t kBigRecQuads = 8;
+constexpr uint32_t kCkMaxSlots = 1u << 24;     // 256 MB
+constexpr uint32_t kCkInitSlots = 1u << 21;    // 32 MB at the first big-triangle frame
 
 // One render request: a target pair, a mesh, a uniform block, and where the
 // request's vertices, tiles and bins sit in the launch's scratch memory.
@@ -178,6 +195,10 @@ struct FrameParams {
     uint32_t busy_map;
     uint32_t *big;         // big triangles (primitive ids), big_cap slots; big_cap 0: off (batches)
     uint32_t big_cap;
+    uint4 *big_rec;        // with ck_cap: per big triangle kBigRecQuads quads, its setup record
+                           // (rec0, rec1) then (primitive, first checkpoint slot or ~0u, 0, 0)
+    uint4 *ck;             // checkpoint pool, ck_cap slots; ck_cap 0: off
+    uint32_t ck_cap;
     TileBin *count_next;
     uint32_t zero_n;
     uint32_t frame;        // frame parity selects the overflow counter and tile queue
@@ -221,6 +242,9 @@ hipError_t noalloc_scratch(int device, void **out);
 // workgroups as p.blk maps them); one-request launches read p.v0.
 hipError_t launch_setup(const FrameParams &p, hipStream_t s, bool batched, uint32_t blocks);
 hipError_t launch_raster(const FrameParams &p, hipStream_t s, bool batched, uint32_t max_groups);
+// Row checkpoints of the big triangles (one-request big-triangle frames with a
+// checkpoint pool), between setup and raster; `groups` workgroups.
+hipError_t launch_span(const FrameParams &p, hipStream_t s, uint32_t groups);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s);
 // The lines path of one request (p.v0): vertex stage + line setup, winner keys
 // initialised, lines walked, then every pixel resolved (colour, depth, clears).

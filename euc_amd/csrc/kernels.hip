@@ -363,9 +363,8 @@ EUCHIP_DEV void pack_rec(const TriRec &R, uint4 q0[kRec0Quads], uint4 q1[kRec1Qu
     q1[1] = make_uint4(__float_as_uint(R.vby[4]), __float_as_uint(R.vby[5]), 0u, 0u);
 }
 
-// o, dx, dy, hz, bounds and flags of record `rid`.
-EUCHIP_DEV void load_rec0(const FrameParams &p, uint32_t rid, TriRec &R) {
-    const uint4 *q = p.rec0 + (size_t)rid * kRec0Quads;
+// o, dx, dy, hz, bounds and flags from a record's first kRec0Quads quads.
+EUCHIP_DEV void load_rec0_at(const uint4 *q, TriRec &R) {
     const uint4 a = q[0], b = q[1], c = q[2], d = q[3];
     R.o[0] = __uint_as_float(a.x), R.o[1] = __uint_as_float(a.y), R.o[2] = __uint_as_float(a.z);
     R.dx[0] = __uint_as_float(a.w), R.dx[1] = __uint_as_float(b.x), R.dx[2] = __uint_as_float(b.y);
@@ -374,12 +373,21 @@ EUCHIP_DEV void load_rec0(const FrameParams &p, uint32_t rid, TriRec &R) {
     R.xmin = d.x, R.xmax = d.y, R.ymin = d.z, R.ymax = d.w & 0x3fffffffu, R.flags = d.w >> 30;
 }
 
-// verts_by_y of record `rid`.
-EUCHIP_DEV void load_rec1(const FrameParams &p, uint32_t rid, TriRec &R) {
-    const uint4 *q = p.rec1 + (size_t)rid * kRec1Quads;
+// o, dx, dy, hz, bounds and flags of record `rid`.
+EUCHIP_DEV void load_rec0(const FrameParams &p, uint32_t rid, TriRec &R) {
+    load_rec0_at(p.rec0 + (size_t)rid * kRec0Quads, R);
+}
+
+// verts_by_y from a record's kRec1Quads secondary quads.
+EUCHIP_DEV void load_rec1_at(const uint4 *q, TriRec &R) {
     const uint4 a = q[0], b = q[1];
     R.vby[0] = __uint_as_float(a.x), R.vby[1] = __uint_as_float(a.y), R.vby[2] = __uint_as_float(a.z);
     R.vby[3] = __uint_as_float(a.w), R.vby[4] = __uint_as_float(b.x), R.vby[5] = __uint_as_float(b.y);
+}
+
+// verts_by_y of record `rid`.
+EUCHIP_DEV void load_rec1(const FrameParams &p, uint32_t rid, TriRec &R) {
+    load_rec1_at(p.rec1 + (size_t)rid * kRec1Quads, R);
 }
 
 // Shaded-vertex ids of primitive t's corners after the winding swap.
@@ -489,7 +497,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
             ny = ((R.ymax - 1u) >> sy) - ty0 + 1u;
             if (p.big_cap && nx * ny > kBigTiles) {  // on the big-triangle list instead of the bins
                 const uint32_t k = atomicAdd(&p.ctr->big[p.frame & 1u][0], 1u);
-                if (k < p.big_cap) p.big[k] = t, nx = ny = 0u;
+                if (k < p.big_cap) {
+                    p.big[k] = t;
+                    if (p.ck_cap) {  // room for its row checkpoints, if the pool has it
+                        const unsigned long long need = (unsigned long long)(R.ymax - R.ymin) * (nx + 1u);
+                        const unsigned long long at = atomicAdd(&p.ctr->ck[p.frame & 1u][0], need);
+                        uint4 q0[kRec0Quads], q1[kRec1Quads];
+                        pack_rec(R, q0, q1);
+                        uint4 *dst = p.big_rec + (size_t)k * kBigRecQuads;
+#pragma unroll
+                        for (uint32_t i = 0; i < kRec0Quads; i++) dst[i] = q0[i];
+#pragma unroll
+                        for (uint32_t i = 0; i < kRec1Quads; i++) dst[kRec0Quads + i] = q1[i];
+                        dst[kRec0Quads + kRec1Quads] = make_uint4(t, at + need <= p.ck_cap ? (uint32_t)at : ~0u, 0u, 0u);
+                    }
+                    nx = ny = 0u;
+                }
             }
             uint4 q0[kRec0Quads], q1[kRec1Quads];
             pack_rec(R, q0, q1);
@@ -775,9 +798,9 @@ struct TileGeom {
     uint32_t X0, Y0, sx, TW, TH;
 };
 
-// The tile's pixels [xs, xe) of one row (index `row` of the first), w_hom at xs
-// given: each pixel's fragment updates its winner key.  Specialised by key mode
-// so the write modes' loops hold no LDS read (nothing then waits on the
+// The tile's pixels [xs, xe) of one row (LDS index pi0 of the first), w_hom at
+// xs given: each pixel's fragment updates its winner key.  Specialised by key
+// mode so the write modes' loops hold no LDS read (nothing then waits on the
 // atomics of earlier pixels).
 template <uint32_t MODE>
 EUCHIP_DEV void walk_row(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t xs, uint32_t xe, uint32_t pi0,
@@ -787,6 +810,37 @@ EUCHIP_DEV void walk_row(const FrameParams &p, const TriRec &R, uint32_t prim, u
         if (frag_z(p, R, w0, w1, w2, z)) {
             const uint32_t pi = pi0 + (x - xs);
             update_key(key + pi, MODE, p.depth_test, z, MODE == 0u ? d0s[pi] : 0.0f, prim);
+        }
+        w0 = w0 + R.dx[0];
+        w1 = w1 + R.dx[1];
+        w2 = w2 + R.dx[2];
+    }
+}
+
+// walk_row for the big-triangle tasks, whose rows are long: coverage, z, z
+// clip and the key of every pixel without branches (z is only used where the
+// pixel is covered), one predicated LDS atomic.
+template <uint32_t MODE>
+EUCHIP_DEV void walk_span(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t xs, uint32_t xe, uint32_t pi0,
+                          float w0, float w1, float w2, uint64_t *key, const float *d0s) {
+    const bool clip = !(R.flags & TRI_ALL_IN) && p.z_clip;
+    const float zlo = p.z_min, zhi = p.z_max;
+    const uint32_t test = p.depth_test;
+    const uint32_t pid = prim + 1u;
+    for (uint32_t x = xs; x < xe; x++) {
+        const float u2 = (w2 - w0) - w1;
+        const float z = (R.hz[0] * w0 + R.hz[1] * w1) + R.hz[2] * u2;
+        bool ok = (w0 >= 0.0f) & (w1 >= 0.0f) & (u2 >= 0.0f);
+        ok = ok & (!clip | ((zlo <= z) & (z <= zhi)));
+        const uint32_t pi = pi0 + (x - xs);
+        if constexpr (MODE == 0u) {
+            const float D0 = d0s[pi];
+            const bool pass = test == DT_LESS ? z < D0 : test == DT_EQUAL ? z == D0 : test == DT_GREATER ? z > D0 : true;
+            if (ok & pass) atomicMax((unsigned long long *)(key + pi), ((uint64_t)pid << 32) | __float_as_uint(z));
+        } else {
+            const uint32_t o = MODE == 1u ? zord(z) : ~zord(z);
+            const uint32_t nz = __float_as_uint(z) == 0x80000000u ? 1u : 0u;
+            if (ok & (z == z)) atomicMin((unsigned long long *)(key + pi), ((uint64_t)o << 32) | (pid << 1) | nz);
         }
         w0 = w0 + R.dx[0];
         w1 = w1 + R.dx[1];
@@ -826,7 +880,7 @@ EUCHIP_DEV void raster_row(const FrameParams &p, const TriRec &R, uint32_t prim,
         w1 = w1 + R.dx[1];
         w2 = w2 + R.dx[2];
     }
-    const uint32_t pi0 = (r << g.sx) + (xs - g.X0);
+    const uint32_t pi0 = r * (g.TW + 1u) + (xs - g.X0);
     if (mode == 1u) walk_row<1u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
     else if (mode == 2u) walk_row<2u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
     else walk_row<0u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
@@ -838,12 +892,14 @@ static_assert(kBatch <= 256u, "a batch is staged one entry per lane");
 // Setup of one bin entry of the current batch, kept in LDS (24 words).
 struct TriLds {
     float o[3], dx[3], dy[3], hz[3], vby[6];
-    uint32_t xmin, xmax, ymin, ymax, flags, prim;
+    uint32_t xmin, xmax, ymin, ymax, flags, prim, ck;
 };
 
 struct TileShared {
-    uint64_t key[256];              // winner key per pixel
-    float d0s[256];                 // depth before the pass (the test's reference)
+    // Per pixel of the tile, rows padded by one entry (pixel i at i + i / TW):
+    // lanes walking the same column of different rows hit different banks.
+    uint64_t key[256 + kMaxTileH];  // winner key
+    float d0s[256 + kMaxTileH];     // depth before the pass (the test's reference)
     TriLds tri[kBatch];             // the batch's triangles
     uint32_t rows_incl[256];        // inclusive prefix sum of their row counts in the tile
     uint8_t owner[256];             // entry of each (entry, row) item of the current 256
@@ -853,17 +909,28 @@ struct TileShared {
 };
 
 // Rows of bin entry `prim` inside the tile, its setup kept in sh.tri[slot].
+// `big` != ~0u: the entry is big triangle `big`, read from its big record
+// (primitive and checkpoints included) instead of the setup records.
 EUCHIP_DEV uint32_t stage_entry(const FrameParams &p, const View &v, TileShared &sh, uint32_t slot, uint32_t prim,
-                                const TileGeom &g) {
+                                uint32_t big, const TileGeom &g) {
     TriRec R;
-    load_rec0(p, v.tri_base + prim, R);
+    uint32_t ck = ~0u;
+    const uint4 *bq = big != ~0u ? p.big_rec + (size_t)big * kBigRecQuads : nullptr;
+    if (bq) {
+        load_rec0_at(bq, R);
+        const uint4 e = bq[kRec0Quads + kRec1Quads];
+        prim = e.x, ck = e.y;
+    } else {
+        load_rec0(p, v.tri_base + prim, R);
+    }
     const uint32_t ry0 = R.ymin > g.Y0 ? R.ymin : g.Y0;
     const uint32_t ry1 = R.ymax < g.Y0 + g.TH ? R.ymax : g.Y0 + g.TH;
     const uint32_t xs = R.xmin > g.X0 ? R.xmin : g.X0;
     const uint32_t xe = R.xmax < g.X0 + g.TW ? R.xmax : g.X0 + g.TW;
     if (ry0 >= ry1 || xs >= xe) return 0u;
     if (!all_rows_boxed(R)) {
-        load_rec1(p, v.tri_base + prim, R);
+        if (bq) load_rec1_at(bq + kRec0Quads, R);
+        else load_rec1(p, v.tri_base + prim, R);
     } else {
 #pragma unroll
         for (int i = 0; i < 6; i++) R.vby[i] = 0.0f;  // boxed rows never read verts_by_y
@@ -873,7 +940,7 @@ EUCHIP_DEV uint32_t stage_entry(const FrameParams &p, const View &v, TileShared 
     for (int i = 0; i < 3; i++) T.o[i] = R.o[i], T.dx[i] = R.dx[i], T.dy[i] = R.dy[i], T.hz[i] = R.hz[i];
 #pragma unroll
     for (int i = 0; i < 6; i++) T.vby[i] = R.vby[i];
-    T.xmin = R.xmin, T.xmax = R.xmax, T.ymin = R.ymin, T.ymax = R.ymax, T.flags = R.flags, T.prim = prim;
+    T.xmin = R.xmin, T.xmax = R.xmax, T.ymin = R.ymin, T.ymax = R.ymax, T.flags = R.flags, T.prim = prim, T.ck = ck;
     return ry1 - ry0;
 }
 
@@ -890,10 +957,27 @@ EUCHIP_DEV uint32_t block_inclusive_sum(TileShared &sh, uint32_t tid, uint32_t v
 // One batch of up to 256 bin entries: each lane stages its entry's setup in LDS,
 // then the (entry, row) pairs of the batch are spread over all lanes so no
 // lane walks much more than one row.
+// A queued row task of the big-triangle path: weights at xs, then entry (8
+// bits), row in the tile (5), xs - X0 (9) and xe - X0 (9).
+EUCHIP_DEV void walk_task(const FrameParams &p, TileShared &sh, const TileGeom &g, uint32_t mode, const uint4 &t) {
+    const uint32_t e = t.w & 0xffu, r = (t.w >> 8) & 31u;
+    const uint32_t xs = g.X0 + ((t.w >> 13) & 511u), xe = g.X0 + (t.w >> 22);
+    const TriLds &T = sh.tri[e];
+    TriRec R;
+#pragma unroll
+    for (int i = 0; i < 3; i++) R.dx[i] = T.dx[i], R.hz[i] = T.hz[i];
+    R.flags = T.flags;
+    const uint32_t pi0 = r * (g.TW + 1u) + (xs - g.X0);
+    const float w0 = __uint_as_float(t.x), w1 = __uint_as_float(t.y), w2 = __uint_as_float(t.z);
+    if (mode == 1u) walk_span<1u>(p, R, T.prim, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+    else if (mode == 2u) walk_span<2u>(p, R, T.prim, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+    else walk_span<0u>(p, R, T.prim, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+}
+
 template <bool OWNER, bool LONG_ROWS>
 EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh, uint32_t tid, bool valid,
-                             uint32_t prim, const TileGeom &g, uint32_t mode) {
-    const uint32_t rows = valid ? stage_entry(p, v, sh, tid, prim, g) : 0u;
+                             uint32_t prim, uint32_t big, const TileGeom &g, uint32_t mode, uint4 *tq) {
+    const uint32_t rows = valid ? stage_entry(p, v, sh, tid, prim, big, g) : 0u;
     const uint32_t incl = block_inclusive_sum(sh, tid, rows);
     const uint32_t excl = incl - rows;
     sh.rows_incl[tid] = incl;
@@ -904,6 +988,78 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
         for (uint32_t k = excl; k < incl && k < 256u; k++) sh.owner[k] = (uint8_t)tid;
     __syncthreads();
     const uint32_t total = sh.rows_incl[255];  // lanes past the batch add nothing
+    if constexpr (LONG_ROWS) {
+        // Big triangles with checkpoints: a round of 256 (entry, row) pairs
+        // loads their spans and checkpoints; the rows that reach into the tile
+        // are queued in LDS (tq, 512 tasks), and every 256 queued tasks are
+        // walked one per lane, so the walks fill whole waves.  Entries without
+        // checkpoints walk from their row start at once.
+        uint32_t nq = 0;
+        for (uint32_t k0 = 0; k0 < total; k0 += 256u) {
+            const uint32_t k = k0 + tid;
+            bool has = false;
+            uint4 task = make_uint4(0u, 0u, 0u, 0u);
+            if (k < total) {
+                uint32_t e;
+                if (OWNER && k < 256u) {
+                    e = sh.owner[k];
+                } else {
+                    uint32_t lo = 0, hi = kBatch - 1u;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (sh.rows_incl[mid] > k) hi = mid;
+                        else lo = mid + 1u;
+                    }
+                    e = lo;
+                }
+                const uint32_t base = e ? sh.rows_incl[e - 1u] : 0u;
+                const TriLds &T = sh.tri[e];
+                const uint32_t ry0 = T.ymin > g.Y0 ? T.ymin : g.Y0;
+                const uint32_t y = ry0 + (k - base);
+                if (T.ck != ~0u) {
+                    const uint32_t rows = T.ymax - T.ymin;
+                    const uint4 *row = p.ck + (size_t)T.ck + (y - T.ymin);
+                    const uint4 sp = row[0];
+                    const uint4 c = row[(size_t)rows * (1u + (g.X0 >> g.sx) - (T.xmin >> g.sx))];
+                    const uint32_t xs = sp.x > g.X0 ? sp.x : g.X0;
+                    const uint32_t xe = sp.y < g.X0 + g.TW ? sp.y : g.X0 + g.TW;
+                    if (xs < xe) {
+                        has = true;
+                        task = make_uint4(c.x, c.y, c.z,
+                                          e | ((y - g.Y0) << 8) | ((xs - g.X0) << 13) | ((xe - g.X0) << 22));
+                    }
+                } else {
+                    TriRec R;
+#pragma unroll
+                    for (int i = 0; i < 3; i++) R.o[i] = T.o[i], R.dx[i] = T.dx[i], R.dy[i] = T.dy[i], R.hz[i] = T.hz[i];
+#pragma unroll
+                    for (int i = 0; i < 6; i++) R.vby[i] = T.vby[i];
+                    R.xmin = T.xmin, R.xmax = T.xmax, R.ymin = T.ymin, R.ymax = T.ymax, R.flags = T.flags;
+                    raster_row<8u>(p, R, T.prim, y, g, mode, sh.key, sh.d0s, sh.band);
+                }
+            }
+            const uint32_t at = block_inclusive_sum(sh, tid, has ? 1u : 0u) - (has ? 1u : 0u);
+            const uint32_t added = sh.wave_rows[0] + sh.wave_rows[1] + sh.wave_rows[2] + sh.wave_rows[3];
+            if (has) tq[nq + at] = task;
+            nq += added;
+            __syncthreads();  // queue written; wave_rows free
+            const bool last = k0 + 256u >= total;
+            while (nq >= 256u || (last && nq)) {
+                const uint32_t take = nq < 256u ? nq : 256u;
+                uint4 mine = make_uint4(0u, 0u, 0u, 0u), moved = mine;
+                if (tid < take) mine = tq[tid];
+                const bool move = tid + 256u < nq;
+                if (move) moved = tq[tid + 256u];
+                __syncthreads();
+                if (move) tq[tid] = moved;
+                nq -= take;
+                if (tid < take) walk_task(p, sh, g, mode, mine);
+                __syncthreads();
+            }
+        }
+        __syncthreads();  // the next batch reuses sh.tri and sh.rows_incl
+        return;
+    }
     for (uint32_t k = tid; k < total; k += 256u) {
         uint32_t e;
         if (OWNER && k < 256u) {
@@ -926,7 +1082,7 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
         for (int i = 0; i < 6; i++) R.vby[i] = T.vby[i];
         R.xmin = T.xmin, R.xmax = T.xmax, R.ymin = T.ymin, R.ymax = T.ymax, R.flags = T.flags;
         const uint32_t ry0 = R.ymin > g.Y0 ? R.ymin : g.Y0;
-        raster_row<LONG_ROWS ? 8u : 4u>(p, R, T.prim, ry0 + (k - base), g, mode, sh.key, sh.d0s, sh.band);
+        raster_row<4u>(p, R, T.prim, ry0 + (k - base), g, mode, sh.key, sh.d0s, sh.band);
     }
     __syncthreads();  // the next batch reuses sh.tri and sh.rows_incl
 }
@@ -940,6 +1096,7 @@ constexpr uint32_t raster_waves(uint32_t sh) {
 template <uint32_t SH, bool BATCH, bool BIG, bool GS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(SH)))) void k_raster(FrameParams p) {
     __shared__ TileShared sh;
+    __shared__ uint4 tq[BIG ? 512 : 1];  // row tasks of the big-triangle path
     const uint32_t tid = threadIdx.x;
     const uint32_t ntiles = p.ntiles;
     const uint32_t par = p.frame & 1u, npar = (p.frame + 1u) & 1u;
@@ -964,6 +1121,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         rp->max_count = p.ctr->max_count[par][0];
         rp->flags = flags;
         rp->pool_used = pool_used0;
+        const unsigned long long ck_used = p.ctr->ck[par][0];
+        rp->ck_used = ck_used < 0xFFFFFFFFull ? (uint32_t)ck_used : 0xFFFFFFFFu;
         if (flags & FLAG_BIN_OVERFLOW) rp->dropped = rp->dropped + 1u;
         __threadfence_system();
         rp->frame = p.frame;
@@ -975,6 +1134,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         p.ctr->max_count[npar][0] = 0u;
         p.ctr->heavy[npar][0] = 0u;
         p.ctr->big[npar][0] = 0u;
+        p.ctr->ck[npar][0] = 0ull;
     }
     if (blockIdx.x < 8u && tid == 0u) {
         p.ctr->next[npar][xcd][0] = 0u;
@@ -1010,8 +1170,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         const size_t pidx = (size_t)py * v.W + px;
         float D0 = 0.0f;
         if (inside && uses_depth) D0 = v.clear_depth_en ? v.clear_depth : v.depth[pidx];
-        sh.key[tid] = init_key(mode, D0);
-        sh.d0s[tid] = D0;
+        const uint32_t lp = tid + (tid >> sx);  // padded LDS index of this lane's pixel
+        sh.key[lp] = init_key(mode, D0);
+        sh.d0s[lp] = D0;
         if (tid < g.TH) {
             uint32_t by0, by1;
             band_of(p, v, g.Y0 + tid, by0, by1);
@@ -1047,21 +1208,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             if (b >= run) break;
             const uint32_t i = b + tid;
             bool valid = false;
-            uint32_t prim = 0;
+            uint32_t prim = 0, big = ~0u;
             if (tid >= kBatch || i >= run) {
             } else if (base) {
                 valid = true, prim = p.pool[base + i];
             } else if (i < nb) {
                 valid = true, prim = bin[i];
-            } else {
-                valid = BIG, prim = BIG ? p.big[i - nb] : 0u;
+            } else if constexpr (BIG) {
+                valid = true;
+                if (p.ck_cap) big = i - nb;
+                else prim = p.big[i - nb];
             }
-            raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, prim, g, mode);
+            raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, prim, big, g, mode, tq);
         }
         uint32_t claimed = 0u;
         if (claim && tid == 0u) claimed = per_xcd + atomicAdd(next, 1u);
         if (inside) {
-            const uint64_t kk = sh.key[tid];
+            const uint64_t kk = sh.key[lp];
             bool has;
             uint32_t prim;
             float z;
@@ -1244,6 +1407,72 @@ static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, u
         }
     }
     launch_raster_g<SH, false>(p, s, batched, groups, cus);
+}
+
+// ---------------------------------------------------------------------------
+// k_span: row checkpoints of the big triangles (layout in internal.hpp).
+// Workgroup s takes big triangles s, s + grid, ...; a lane takes a row: the row
+// span (triangles.rs:219-254, with the row's band), then the walk from x0
+// (triangles.rs:257-301, the same w += dx additions as the raster's walk),
+// storing the weights where the span enters each tile column.  Only the
+// additions: coverage, z and the keys stay in the raster pass.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_span(FrameParams p) {
+    const uint32_t par = p.frame & 1u;
+    uint32_t nbig = __builtin_amdgcn_readfirstlane(p.ctr->big[par][0]);
+    nbig = nbig < p.big_cap ? nbig : p.big_cap;
+    const View &v = p.v0;
+    const uint32_t sx = p.tile_sx;
+    for (uint32_t s = blockIdx.x; s < nbig; s += gridDim.x) {
+        // the big record: setup record and checkpoint slot, loaded together
+        const uint4 *bq = p.big_rec + (size_t)s * kBigRecQuads;
+        TriRec R;
+        load_rec0_at(bq, R);
+        load_rec1_at(bq + kRec0Quads, R);
+        const uint32_t ck = __builtin_amdgcn_readfirstlane(bq[kRec0Quads + kRec1Quads].y);
+        if (ck == ~0u) continue;
+        const uint32_t col0 = R.xmin >> sx;
+        const uint32_t rows = R.ymax - R.ymin;
+        for (uint32_t r = threadIdx.x; r < rows; r += 256u) {
+            const uint32_t y = R.ymin + r;
+            uint32_t by0, by1, x0, x1;
+            band_of(p, v, y, by0, by1);
+            row_span(R, y, by0, by1, x0, x1);
+            uint4 *row = p.ck + (size_t)ck + r;
+            row[0] = make_uint4(x0, x1, 0u, 0u);
+            if (x0 >= x1) continue;
+            const float Y = (float)y, X0 = (float)x0;
+            float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
+            float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
+            float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
+            for (uint32_t x = x0;;) {
+                const uint32_t c = x >> sx;
+                row[(size_t)rows * (1u + c - col0)] =
+                    make_uint4(__float_as_uint(w0), __float_as_uint(w1), __float_as_uint(w2), 0u);
+                const uint32_t xn = (c + 1u) << sx;
+                if (xn >= x1) break;
+                for (; x + 8u <= xn; x += 8u) {
+#pragma unroll
+                    for (uint32_t q = 0; q < 8u; q++) {
+                        w0 = w0 + R.dx[0];
+                        w1 = w1 + R.dx[1];
+                        w2 = w2 + R.dx[2];
+                    }
+                }
+                for (; x < xn; x++) {
+                    w0 = w0 + R.dx[0];
+                    w1 = w1 + R.dx[1];
+                    w2 = w2 + R.dx[2];
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_span(const FrameParams &p, hipStream_t s, uint32_t groups) {
+    if (!p.big_cap || !p.ck_cap || groups == 0u) return hipSuccess;
+    hipLaunchKernelGGL(k_span, dim3(groups), dim3(256), 0, s, p);
+    return hipGetLastError();
 }
 
 // Workgroups of the tile loop: a multiple of the 8 XCDs, no more than the

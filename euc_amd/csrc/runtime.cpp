@@ -68,6 +68,9 @@ struct euchip_ctx {
     DevArray<uint32_t> heavy;   // heavy-tile list
     DevArray<uint32_t> busy;    // busy-tile lists (one per XCD tile range)
     DevArray<uint32_t> big;     // big-triangle list
+    DevArray<uint4> big_rec;    // per big triangle: setup record and first checkpoint slot
+    DevArray<uint4> ck;         // row checkpoints of big triangles
+    size_t ck_want = 0;         // checkpoint slots the next big-triangle frames should have (from the reports)
     DevArray<uint32_t> lrec;    // line records
     DevArray<uint64_t> lkey;    // per-pixel winner keys of a lines frame
     std::vector<Deferred> deferred;
@@ -487,6 +490,8 @@ void absorb_report(euchip_ctx *ctx) {
         ctx->tile_cap = c;
     }
     ctx->pool_want = std::max<size_t>(ctx->pool_want, (size_t)used + used / 4);
+    const size_t ck = r->ck_used;
+    ctx->ck_want = std::min<size_t>(std::max<size_t>(ctx->ck_want, ck + ck / 4), kCkMaxSlots);
 }
 
 // render() of one request of lines (lines.rs:15-121) on the context's stream.
@@ -569,6 +574,14 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     if ((s = ensure(ctx, ctx->busy, 8 * ntiles, "busy tiles", ctx->stream))) return s;
     const size_t big_cap = std::min<size_t>(std::max<size_t>(P.n_tris, 1), kBigCap);
     if ((s = ensure(ctx, ctx->big, big_cap, "big triangles", ctx->stream))) return s;
+    // Row checkpoints (big-triangle frames): optional, so a pool the budget
+    // cannot hold leaves the big triangles on the walk from the row start.
+    bool ck_on = false;
+    if (big_mode) {
+        ck_on = ensure(ctx, ctx->big_rec, big_cap * kBigRecQuads, "big triangles", ctx->stream) == EUCHIP_OK &&
+                ensure(ctx, ctx->ck, std::max<size_t>(ctx->ck_want, kCkInitSlots), "row checkpoints",
+                       ctx->stream) == EUCHIP_OK;
+    }
 
     for (int attempt = 0;; attempt++) {
         const uint32_t cap = frame_tile_cap(ctx->tile_cap, ntiles, P.n_tris);
@@ -588,6 +601,12 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.busy_map = BUSY_BLOCKS;
         p.big = ctx->big.ptr;
         p.big_cap = big_mode ? (uint32_t)std::min<size_t>(ctx->big.cap, kBigCap) : 0u;
+        if (ck_on) {
+            p.big_cap = (uint32_t)std::min<size_t>(p.big_cap, ctx->big_rec.cap / kBigRecQuads);
+            p.big_rec = ctx->big_rec.ptr;
+            p.ck = ctx->ck.ptr;
+            p.ck_cap = (uint32_t)std::min<size_t>(ctx->ck.cap, kCkMaxSlots);
+        }
         p.bins = ctx->bins.ptr;
         p.v0.tile_cap = cap;
         p.v0.bin_base = 0;
@@ -597,6 +616,10 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.heavy_cap = (uint32_t)std::min<size_t>(ctx->heavy.cap, 0xFFFFFFF0ull);
         p.frame = ctx->frame++;
         if ((s = timed(ctx, "setup", [&] { return launch_setup(p, ctx->stream, false, 0); }))) return s;
+        if (p.ck_cap) {
+            const uint32_t groups = (uint32_t)std::min<size_t>(p.big_cap, 2048);
+            if ((s = timed(ctx, "span", [&] { return launch_span(p, ctx->stream, groups); }))) return s;
+        }
         if ((s = timed(ctx, "raster", [&] { return launch_raster(p, ctx->stream, false, ctx->max_groups); }))) return s;
         ctx->counts_used[npar] = 0;
         ctx->counts_used[par] = (uint32_t)ntiles;
@@ -974,6 +997,8 @@ euchip_status euchip_ctx_destroy(euchip_ctx *ctx) {
     release(ctx, ctx->heavy);
     release(ctx, ctx->busy);
     release(ctx, ctx->big);
+    release(ctx, ctx->big_rec);
+    release(ctx, ctx->ck);
     release(ctx, ctx->lrec);
     release(ctx, ctx->lkey);
 
