@@ -47,7 +47,9 @@ enum TriFlags : uint32_t {
 //                          large triangles), 2 words unused
 constexpr uint32_t kRec0Quads = 4, kRec1Quads = 2;
 
-enum FrameFlags : uint32_t { FLAG_BIN_OVERFLOW = 1u };
+// FLAG_CK_MISS: a big triangle found no room in the checkpoint pool; the
+// frame's big triangles then walk their rows from the row starts.
+enum FrameFlags : uint32_t { FLAG_BIN_OVERFLOW = 1u, FLAG_CK_MISS = 2u };
 
 // Device counters of one launch stream, per frame parity f & 1: setup of frame f
 // counts into the [f & 1] entries, its raster pass reads them and zeroes the
@@ -109,15 +111,23 @@ constexpr uint32_t kBigCap = 4096;
 // row start x0, triangles.rs:257-301) reaches a tile only after every pixel
 // left of it; a pass between setup and raster walks each row of each big
 // triangle once and keeps, per tile column the row's span touches, the weights
-// at the first pixel of the span in that column.  A big triangle with
-// `rows` rows and `cols` tile columns in its box takes rows * (cols + 1)
-// 16-byte slots of the checkpoint pool, column-major (slot k * rows + r, so
-// the lanes of a wave, on consecutive rows, touch consecutive slots): k = 0
-// holds (x0, x1) of row r's span, k = 1 + c the weights (w0, w1, w2) at
-// max(x0, first pixel of box column c).
-// A triangle the pool cannot hold keeps the walk from the row start.
+// at the first pixel of the span in that column.  A big triangle with `rows`
+// rows and `cols` tile columns in its box takes rows * cols 16-byte slots of
+// the checkpoint pool, column-major (slot (c - col0) * rows + r, so lanes on
+// consecutive rows touch consecutive slots): (w0, w1, w2) at xs, then
+// (xs - X0) | (xe - X0) << 16, the row's span inside the column that starts at
+// X0.  Per (tile, big triangle) one 8-byte entry of cktab (tile * mstride +
+// slot) holds, when some row of the tile has such a span, the mask of those
+// rows (bits 32-39) and the slot of the tile's row 0 plus 8 (bits 0-31, so
+// row r's slot is the entry - 8 + r): the raster pass reads its tile's
+// entries, walks the rows from their checkpoints and clears the entries for
+// the next frame.  Tiles are then at most 8 rows high.  A frame in which some
+// big triangle finds no room in the pool (FLAG_CK_MISS) walks all its big
+// triangles from the row starts.
+constexpr uint32_t kSpanRows = 64;             // rows per item of the checkpoint pass (one wave)
+constexpr uint32_t kCkNone = 0x3fffffffu;      // big record: no checkpoint slots
 constexpr uint32_t kBigRecQuads = 8;
-constexpr uint32_t kCkMaxSlots = 1u << 24;     // 256 MB
+constexpr uint32_t kCkMaxSlots = 1u << 24;     // 256 MB (below kCkNone)
 constexpr uint32_t kCkInitSlots = 1u << 21;    // 32 MB at the first big-triangle frame
 
 // One render request: a target pair, a mesh, a uniform block, and where the
@@ -195,10 +205,13 @@ struct FrameParams {
     uint32_t busy_map;
     uint32_t *big;         // big triangles (primitive ids), big_cap slots; big_cap 0: off (batches)
     uint32_t big_cap;
-    uint4 *big_rec;        // with ck_cap: per big triangle kBigRecQuads quads, its setup record
-                           // (rec0, rec1) then (primitive, first checkpoint slot or ~0u, 0, 0)
+    uint4 *big_rec;        // with ck_cap: per big triangle kBigRecQuads quads: its setup record
+                           // (rec0, rec1), then (primitive, first checkpoint slot (~0u: none),
+                           // dx0, dx1), (dx2, hz0, hz1, hz2)
     uint4 *ck;             // checkpoint pool, ck_cap slots; ck_cap 0: off
     uint32_t ck_cap;
+    unsigned long long *cktab;  // row masks and slots, ntiles x mstride entries (zero between frames)
+    uint32_t mstride;
     TileBin *count_next;
     uint32_t zero_n;
     uint32_t frame;        // frame parity selects the overflow counter and tile queue
@@ -243,8 +256,8 @@ hipError_t noalloc_scratch(int device, void **out);
 hipError_t launch_setup(const FrameParams &p, hipStream_t s, bool batched, uint32_t blocks);
 hipError_t launch_raster(const FrameParams &p, hipStream_t s, bool batched, uint32_t max_groups);
 // Row checkpoints of the big triangles (one-request big-triangle frames with a
-// checkpoint pool), between setup and raster; `groups` workgroups.
-hipError_t launch_span(const FrameParams &p, hipStream_t s, uint32_t groups);
+// checkpoint pool), between setup and raster.
+hipError_t launch_span(const FrameParams &p, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s);
 // The lines path of one request (p.v0): vertex stage + line setup, winner keys
 // initialised, lines walked, then every pixel resolved (colour, depth, clears).

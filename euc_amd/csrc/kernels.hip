@@ -500,8 +500,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                 if (k < p.big_cap) {
                     p.big[k] = t;
                     if (p.ck_cap) {  // room for its row checkpoints, if the pool has it
-                        const unsigned long long need = (unsigned long long)(R.ymax - R.ymin) * (nx + 1u);
+                        const unsigned long long need = (unsigned long long)(R.ymax - R.ymin) * nx;
                         const unsigned long long at = atomicAdd(&p.ctr->ck[p.frame & 1u][0], need);
+                        if (at + need > p.ck_cap) atomicOr(&p.ctr->flags[p.frame & 1u][0], FLAG_CK_MISS);
                         uint4 q0[kRec0Quads], q1[kRec1Quads];
                         pack_rec(R, q0, q1);
                         uint4 *dst = p.big_rec + (size_t)k * kBigRecQuads;
@@ -509,7 +510,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                         for (uint32_t i = 0; i < kRec0Quads; i++) dst[i] = q0[i];
 #pragma unroll
                         for (uint32_t i = 0; i < kRec1Quads; i++) dst[kRec0Quads + i] = q1[i];
-                        dst[kRec0Quads + kRec1Quads] = make_uint4(t, at + need <= p.ck_cap ? (uint32_t)at : ~0u, 0u, 0u);
+                        // what the raster pass's checkpoint tasks read: primitive, first
+                        // slot | flags << 30, dx, hz
+                        dst[6] = make_uint4(t, (at + need <= p.ck_cap ? (uint32_t)at : kCkNone) | (R.flags << 30),
+                                            __float_as_uint(R.dx[0]), __float_as_uint(R.dx[1]));
+                        dst[7] = make_uint4(__float_as_uint(R.dx[2]), __float_as_uint(R.hz[0]), __float_as_uint(R.hz[1]),
+                                            __float_as_uint(R.hz[2]));
                     }
                     nx = ny = 0u;
                 }
@@ -892,7 +898,7 @@ static_assert(kBatch <= 256u, "a batch is staged one entry per lane");
 // Setup of one bin entry of the current batch, kept in LDS (24 words).
 struct TriLds {
     float o[3], dx[3], dy[3], hz[3], vby[6];
-    uint32_t xmin, xmax, ymin, ymax, flags, prim, ck;
+    uint32_t xmin, xmax, ymin, ymax, flags, prim;
 };
 
 struct TileShared {
@@ -914,12 +920,10 @@ struct TileShared {
 EUCHIP_DEV uint32_t stage_entry(const FrameParams &p, const View &v, TileShared &sh, uint32_t slot, uint32_t prim,
                                 uint32_t big, const TileGeom &g) {
     TriRec R;
-    uint32_t ck = ~0u;
     const uint4 *bq = big != ~0u ? p.big_rec + (size_t)big * kBigRecQuads : nullptr;
     if (bq) {
         load_rec0_at(bq, R);
-        const uint4 e = bq[kRec0Quads + kRec1Quads];
-        prim = e.x, ck = e.y;
+        prim = bq[6].x;
     } else {
         load_rec0(p, v.tri_base + prim, R);
     }
@@ -940,7 +944,7 @@ EUCHIP_DEV uint32_t stage_entry(const FrameParams &p, const View &v, TileShared 
     for (int i = 0; i < 3; i++) T.o[i] = R.o[i], T.dx[i] = R.dx[i], T.dy[i] = R.dy[i], T.hz[i] = R.hz[i];
 #pragma unroll
     for (int i = 0; i < 6; i++) T.vby[i] = R.vby[i];
-    T.xmin = R.xmin, T.xmax = R.xmax, T.ymin = R.ymin, T.ymax = R.ymax, T.flags = R.flags, T.prim = prim, T.ck = ck;
+    T.xmin = R.xmin, T.xmax = R.xmax, T.ymin = R.ymin, T.ymax = R.ymax, T.flags = R.flags, T.prim = prim;
     return ry1 - ry0;
 }
 
@@ -957,26 +961,9 @@ EUCHIP_DEV uint32_t block_inclusive_sum(TileShared &sh, uint32_t tid, uint32_t v
 // One batch of up to 256 bin entries: each lane stages its entry's setup in LDS,
 // then the (entry, row) pairs of the batch are spread over all lanes so no
 // lane walks much more than one row.
-// A queued row task of the big-triangle path: weights at xs, then entry (8
-// bits), row in the tile (5), xs - X0 (9) and xe - X0 (9).
-EUCHIP_DEV void walk_task(const FrameParams &p, TileShared &sh, const TileGeom &g, uint32_t mode, const uint4 &t) {
-    const uint32_t e = t.w & 0xffu, r = (t.w >> 8) & 31u;
-    const uint32_t xs = g.X0 + ((t.w >> 13) & 511u), xe = g.X0 + (t.w >> 22);
-    const TriLds &T = sh.tri[e];
-    TriRec R;
-#pragma unroll
-    for (int i = 0; i < 3; i++) R.dx[i] = T.dx[i], R.hz[i] = T.hz[i];
-    R.flags = T.flags;
-    const uint32_t pi0 = r * (g.TW + 1u) + (xs - g.X0);
-    const float w0 = __uint_as_float(t.x), w1 = __uint_as_float(t.y), w2 = __uint_as_float(t.z);
-    if (mode == 1u) walk_span<1u>(p, R, T.prim, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-    else if (mode == 2u) walk_span<2u>(p, R, T.prim, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-    else walk_span<0u>(p, R, T.prim, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-}
-
 template <bool OWNER, bool LONG_ROWS>
 EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh, uint32_t tid, bool valid,
-                             uint32_t prim, uint32_t big, const TileGeom &g, uint32_t mode, uint4 *tq) {
+                             uint32_t prim, uint32_t big, const TileGeom &g, uint32_t mode) {
     const uint32_t rows = valid ? stage_entry(p, v, sh, tid, prim, big, g) : 0u;
     const uint32_t incl = block_inclusive_sum(sh, tid, rows);
     const uint32_t excl = incl - rows;
@@ -988,78 +975,6 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
         for (uint32_t k = excl; k < incl && k < 256u; k++) sh.owner[k] = (uint8_t)tid;
     __syncthreads();
     const uint32_t total = sh.rows_incl[255];  // lanes past the batch add nothing
-    if constexpr (LONG_ROWS) {
-        // Big triangles with checkpoints: a round of 256 (entry, row) pairs
-        // loads their spans and checkpoints; the rows that reach into the tile
-        // are queued in LDS (tq, 512 tasks), and every 256 queued tasks are
-        // walked one per lane, so the walks fill whole waves.  Entries without
-        // checkpoints walk from their row start at once.
-        uint32_t nq = 0;
-        for (uint32_t k0 = 0; k0 < total; k0 += 256u) {
-            const uint32_t k = k0 + tid;
-            bool has = false;
-            uint4 task = make_uint4(0u, 0u, 0u, 0u);
-            if (k < total) {
-                uint32_t e;
-                if (OWNER && k < 256u) {
-                    e = sh.owner[k];
-                } else {
-                    uint32_t lo = 0, hi = kBatch - 1u;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (sh.rows_incl[mid] > k) hi = mid;
-                        else lo = mid + 1u;
-                    }
-                    e = lo;
-                }
-                const uint32_t base = e ? sh.rows_incl[e - 1u] : 0u;
-                const TriLds &T = sh.tri[e];
-                const uint32_t ry0 = T.ymin > g.Y0 ? T.ymin : g.Y0;
-                const uint32_t y = ry0 + (k - base);
-                if (T.ck != ~0u) {
-                    const uint32_t rows = T.ymax - T.ymin;
-                    const uint4 *row = p.ck + (size_t)T.ck + (y - T.ymin);
-                    const uint4 sp = row[0];
-                    const uint4 c = row[(size_t)rows * (1u + (g.X0 >> g.sx) - (T.xmin >> g.sx))];
-                    const uint32_t xs = sp.x > g.X0 ? sp.x : g.X0;
-                    const uint32_t xe = sp.y < g.X0 + g.TW ? sp.y : g.X0 + g.TW;
-                    if (xs < xe) {
-                        has = true;
-                        task = make_uint4(c.x, c.y, c.z,
-                                          e | ((y - g.Y0) << 8) | ((xs - g.X0) << 13) | ((xe - g.X0) << 22));
-                    }
-                } else {
-                    TriRec R;
-#pragma unroll
-                    for (int i = 0; i < 3; i++) R.o[i] = T.o[i], R.dx[i] = T.dx[i], R.dy[i] = T.dy[i], R.hz[i] = T.hz[i];
-#pragma unroll
-                    for (int i = 0; i < 6; i++) R.vby[i] = T.vby[i];
-                    R.xmin = T.xmin, R.xmax = T.xmax, R.ymin = T.ymin, R.ymax = T.ymax, R.flags = T.flags;
-                    raster_row<8u>(p, R, T.prim, y, g, mode, sh.key, sh.d0s, sh.band);
-                }
-            }
-            const uint32_t at = block_inclusive_sum(sh, tid, has ? 1u : 0u) - (has ? 1u : 0u);
-            const uint32_t added = sh.wave_rows[0] + sh.wave_rows[1] + sh.wave_rows[2] + sh.wave_rows[3];
-            if (has) tq[nq + at] = task;
-            nq += added;
-            __syncthreads();  // queue written; wave_rows free
-            const bool last = k0 + 256u >= total;
-            while (nq >= 256u || (last && nq)) {
-                const uint32_t take = nq < 256u ? nq : 256u;
-                uint4 mine = make_uint4(0u, 0u, 0u, 0u), moved = mine;
-                if (tid < take) mine = tq[tid];
-                const bool move = tid + 256u < nq;
-                if (move) moved = tq[tid + 256u];
-                __syncthreads();
-                if (move) tq[tid] = moved;
-                nq -= take;
-                if (tid < take) walk_task(p, sh, g, mode, mine);
-                __syncthreads();
-            }
-        }
-        __syncthreads();  // the next batch reuses sh.tri and sh.rows_incl
-        return;
-    }
     for (uint32_t k = tid; k < total; k += 256u) {
         uint32_t e;
         if (OWNER && k < 256u) {
@@ -1082,9 +997,63 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
         for (int i = 0; i < 6; i++) R.vby[i] = T.vby[i];
         R.xmin = T.xmin, R.xmax = T.xmax, R.ymin = T.ymin, R.ymax = T.ymax, R.flags = T.flags;
         const uint32_t ry0 = R.ymin > g.Y0 ? R.ymin : g.Y0;
-        raster_row<4u>(p, R, T.prim, ry0 + (k - base), g, mode, sh.key, sh.d0s, sh.band);
+        raster_row<LONG_ROWS ? 8u : 4u>(p, R, T.prim, ry0 + (k - base), g, mode, sh.key, sh.d0s, sh.band);
     }
     __syncthreads();  // the next batch reuses sh.tri and sh.rows_incl
+}
+
+// The big triangles of `tile` from their row checkpoints, 256 at a time: the
+// tile's cktab entries (cleared here for the next frame; one coalesced 8-byte
+// load per triangle) name the rows whose span reaches into the tile and their
+// checkpoint slots; a prefix sum over the row counts spreads those rows over
+// the lanes, and each lane walks one row from its checkpoint (weights and the
+// span inside the tile's column), with the triangle's dx, hz, flags and
+// primitive from its big record (two loads beside the checkpoint's).
+EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, uint32_t tile, uint32_t nbig,
+                          const TileGeom &g, uint32_t mode) {
+    uint32_t *vtab = reinterpret_cast<uint32_t *>(sh.tri);  // slot of the tile's row 0, + 8
+    unsigned long long *trow = p.cktab + (size_t)tile * p.mstride;
+    for (uint32_t s0 = 0; s0 < nbig; s0 += 256u) {
+        const uint32_t s = s0 + tid;
+        unsigned long long e = 0ull;
+        if (s < nbig) {
+            e = trow[s];
+            if (e) trow[s] = 0ull;
+        }
+        const uint32_t m = (uint32_t)(e >> 32) & 0xffu;
+        const uint32_t incl = block_inclusive_sum(sh, tid, __builtin_popcount(m));
+        sh.rows_incl[tid] = incl;
+        sh.owner[tid] = (uint8_t)m;
+        vtab[tid] = (uint32_t)e;
+        __syncthreads();
+        const uint32_t total = sh.rows_incl[255];
+        for (uint32_t k = tid; k < total; k += 256u) {
+            uint32_t lo = 0, hi = 255u;  // first slot whose inclusive count exceeds k
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sh.rows_incl[mid] > k) hi = mid;
+                else lo = mid + 1u;
+            }
+            uint32_t j = k - (lo ? sh.rows_incl[lo - 1u] : 0u);
+            uint32_t bits = sh.owner[lo];
+            for (; j; j--) bits &= bits - 1u;  // drop the j lowest rows
+            const uint32_t r = __builtin_ctz(bits);
+            const uint4 *q = p.big_rec + (size_t)(s0 + lo) * kBigRecQuads;
+            const uint4 c = p.ck[(size_t)vtab[lo] - 8u + r];
+            const uint4 e6 = q[6], e7 = q[7];
+            const uint32_t xs = g.X0 + (c.w & 0xffffu), xe = g.X0 + (c.w >> 16);
+            TriRec R;
+            R.dx[0] = __uint_as_float(e6.z), R.dx[1] = __uint_as_float(e6.w), R.dx[2] = __uint_as_float(e7.x);
+            R.hz[0] = __uint_as_float(e7.y), R.hz[1] = __uint_as_float(e7.z), R.hz[2] = __uint_as_float(e7.w);
+            R.flags = e6.y >> 30;
+            const uint32_t pi0 = r * (g.TW + 1u) + (xs - g.X0);
+            const float w0 = __uint_as_float(c.x), w1 = __uint_as_float(c.y), w2 = __uint_as_float(c.z);
+            if (mode == 1u) walk_span<1u>(p, R, e6.x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+            else if (mode == 2u) walk_span<2u>(p, R, e6.x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+            else walk_span<0u>(p, R, e6.x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+        }
+        __syncthreads();  // the next 256 reuse the counts, masks and slots
+    }
 }
 
 // Waves per SIMD the raster pass is compiled for (measured per shader): latency
@@ -1096,7 +1065,6 @@ constexpr uint32_t raster_waves(uint32_t sh) {
 template <uint32_t SH, bool BATCH, bool BIG, bool GS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(SH)))) void k_raster(FrameParams p) {
     __shared__ TileShared sh;
-    __shared__ uint4 tq[BIG ? 512 : 1];  // row tasks of the big-triangle path
     const uint32_t tid = threadIdx.x;
     const uint32_t ntiles = p.ntiles;
     const uint32_t par = p.frame & 1u, npar = (p.frame + 1u) & 1u;
@@ -1150,6 +1118,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
                         : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
     const uint32_t sx = p.tile_sx, sy = 8u - sx;
     uint32_t *next = &p.ctr->next[par][xcd][0];
+    // big triangles from their row checkpoints (else walked from the row starts)
+    const bool ckp = BIG && p.ck_cap && !(flags & FLAG_CK_MISS);
 
     // One tile with `cnt` bin entries and overflow list `head`: winner keys in
     // LDS, then the resolve.
@@ -1189,7 +1159,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         // most pool_used / 3 segments whose every read stays below pool_used
         // ends even on a damaged list.  One loop, so one copy of raster_batch.
         const uint32_t nb = cnt < cap ? cnt : cap;
-        uint32_t run = nb + nbig;           // entries of the current run
+        uint32_t run = nb + (ckp ? 0u : nbig);  // entries of the current run
         uint32_t base = 0u;                 // pool index of the current segment's entries (0: bin + big)
         uint32_t seg = cnt > cap ? head : 0u, guard = 0u;
         for (uint32_t b = 0;; b += kBatch) {
@@ -1219,8 +1189,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
                 if (p.ck_cap) big = i - nb;
                 else prim = p.big[i - nb];
             }
-            raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, prim, big, g, mode, tq);
+            raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, prim, big, g, mode);
         }
+        if (BIG && ckp && nbig) big_tasks(p, sh, tid, tile, nbig, g, mode);
         uint32_t claimed = 0u;
         if (claim && tid == 0u) claimed = per_xcd + atomicAdd(next, 1u);
         if (inside) {
@@ -1410,68 +1381,103 @@ static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, u
 }
 
 // ---------------------------------------------------------------------------
-// k_span: row checkpoints of the big triangles (layout in internal.hpp).
-// Workgroup s takes big triangles s, s + grid, ...; a lane takes a row: the row
-// span (triangles.rs:219-254, with the row's band), then the walk from x0
-// (triangles.rs:257-301, the same w += dx additions as the raster's walk),
-// storing the weights where the span enters each tile column.  Only the
-// additions: coverage, z and the keys stay in the raster pass.
+// k_span: row checkpoints and cktab entries of the big triangles (layout in
+// internal.hpp).  An item is 64 rows of one big triangle, counted from its
+// first row rounded down to a tile row (so every tile row belongs to one
+// item); workgroup b takes triangle b % nslots and its wave w item
+// 4 * (b / nslots) + w.  A lane takes a row: the row span (triangles.rs:219-254,
+// with the row's band), a bit in the wave's LDS mask table for every tile
+// column the span touches, then the walk from x0 (triangles.rs:257-301, the
+// same w += dx additions as the raster's walk) storing the weights where the
+// span enters each column.  The wave then writes the entries of its tiles.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_span(FrameParams p) {
+__global__ __launch_bounds__(256) void k_span(FrameParams p, uint32_t nslots, uint32_t parts) {
+    extern __shared__ uint32_t span_lds[];  // per wave: (kSpanRows >> sy) x ntx mask bytes
     const uint32_t par = p.frame & 1u;
+    const uint32_t flags = p.ctr->flags[par][0];
+    if (flags & (FLAG_BIN_OVERFLOW | FLAG_CK_MISS)) return;
     uint32_t nbig = __builtin_amdgcn_readfirstlane(p.ctr->big[par][0]);
     nbig = nbig < p.big_cap ? nbig : p.big_cap;
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t s = blockIdx.x % nslots, part = (blockIdx.x / nslots) * 4u + wv;
+    if (s >= nbig || part >= parts) return;
     const View &v = p.v0;
-    const uint32_t sx = p.tile_sx;
-    for (uint32_t s = blockIdx.x; s < nbig; s += gridDim.x) {
-        // the big record: setup record and checkpoint slot, loaded together
-        const uint4 *bq = p.big_rec + (size_t)s * kBigRecQuads;
-        TriRec R;
-        load_rec0_at(bq, R);
-        load_rec1_at(bq + kRec0Quads, R);
-        const uint32_t ck = __builtin_amdgcn_readfirstlane(bq[kRec0Quads + kRec1Quads].y);
-        if (ck == ~0u) continue;
-        const uint32_t col0 = R.xmin >> sx;
-        const uint32_t rows = R.ymax - R.ymin;
-        for (uint32_t r = threadIdx.x; r < rows; r += 256u) {
-            const uint32_t y = R.ymin + r;
-            uint32_t by0, by1, x0, x1;
-            band_of(p, v, y, by0, by1);
-            row_span(R, y, by0, by1, x0, x1);
-            uint4 *row = p.ck + (size_t)ck + r;
-            row[0] = make_uint4(x0, x1, 0u, 0u);
-            if (x0 >= x1) continue;
-            const float Y = (float)y, X0 = (float)x0;
-            float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
-            float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
-            float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
-            for (uint32_t x = x0;;) {
-                const uint32_t c = x >> sx;
-                row[(size_t)rows * (1u + c - col0)] =
-                    make_uint4(__float_as_uint(w0), __float_as_uint(w1), __float_as_uint(w2), 0u);
-                const uint32_t xn = (c + 1u) << sx;
-                if (xn >= x1) break;
-                for (; x + 8u <= xn; x += 8u) {
+    const uint32_t sx = p.tile_sx, sy = 8u - sx, TH = 1u << sy;
+    const uint32_t groups = kSpanRows >> sy;
+    uint32_t *mw = span_lds + wv * ((groups * v.ntx + 3u) >> 2);
+    const uint8_t *mb = reinterpret_cast<const uint8_t *>(mw);
+    const uint4 *bq = p.big_rec + (size_t)s * kBigRecQuads;
+    TriRec R;
+    load_rec0_at(bq, R);
+    load_rec1_at(bq + kRec0Quads, R);
+    const uint32_t ck = __builtin_amdgcn_readfirstlane(bq[6].y) & kCkNone;
+    const uint32_t ybase = ((R.ymin >> sy) << sy) + part * kSpanRows;
+    if (ck == kCkNone || ybase >= R.ymax) return;
+    const uint32_t col0 = R.xmin >> sx;
+    const uint32_t cols = ((R.xmax - 1u) >> sx) - col0 + 1u;
+    const uint32_t rows = R.ymax - R.ymin;
+    for (uint32_t i = lane; i < ((groups * cols + 3u) >> 2); i += 64u) mw[i] = 0u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t y = ybase + lane;
+    uint32_t x0 = 0, x1 = 0;
+    if (y >= R.ymin && y < R.ymax) {
+        uint32_t by0, by1;
+        band_of(p, v, y, by0, by1);
+        row_span(R, y, by0, by1, x0, x1);
+    }
+    if (x0 < x1) {
+        const uint32_t at0 = (lane >> sy) * cols - col0, bit = 1u << (y & (TH - 1u));
+        for (uint32_t c = x0 >> sx; c <= ((x1 - 1u) >> sx); c++) {
+            const uint32_t at = at0 + c;
+            atomicOr(&mw[at >> 2], bit << ((at & 3u) * 8u));
+        }
+        const float Y = (float)y, X0 = (float)x0;
+        float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
+        float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
+        float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
+        uint4 *row = p.ck + (size_t)ck + (y - R.ymin);
+        for (uint32_t x = x0;;) {
+            const uint32_t c = x >> sx, xc = c << sx;
+            const uint32_t xe = xc + (1u << sx) < x1 ? xc + (1u << sx) : x1;
+            row[(size_t)rows * (c - col0)] = make_uint4(__float_as_uint(w0), __float_as_uint(w1),
+                                                        __float_as_uint(w2), (x - xc) | ((xe - xc) << 16));
+            if (xe >= x1) break;
+            for (; x + 8u <= xe; x += 8u) {
 #pragma unroll
-                    for (uint32_t q = 0; q < 8u; q++) {
-                        w0 = w0 + R.dx[0];
-                        w1 = w1 + R.dx[1];
-                        w2 = w2 + R.dx[2];
-                    }
-                }
-                for (; x < xn; x++) {
+                for (uint32_t q = 0; q < 8u; q++) {
                     w0 = w0 + R.dx[0];
                     w1 = w1 + R.dx[1];
                     w2 = w2 + R.dx[2];
                 }
             }
+            for (; x < xe; x++) {
+                w0 = w0 + R.dx[0];
+                w1 = w1 + R.dx[1];
+                w2 = w2 + R.dx[2];
+            }
         }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // the entries of this item's tiles (every tile row of it is this item's)
+    const uint32_t ty0 = ybase >> sy;
+    for (uint32_t i = lane; i < groups * cols; i += 64u) {
+        const uint32_t b = mb[i];
+        if (b == 0u) continue;
+        const uint32_t gi = i / cols, c = i - gi * cols;
+        const uint32_t slot8 = ck + c * rows + ((ty0 + gi) << sy) + 8u - R.ymin;  // tile row 0's slot + 8
+        p.cktab[((size_t)(ty0 + gi) * v.ntx + col0 + c) * p.mstride + s] = ((unsigned long long)b << 32) | slot8;
     }
 }
 
-hipError_t launch_span(const FrameParams &p, hipStream_t s, uint32_t groups) {
-    if (!p.big_cap || !p.ck_cap || groups == 0u) return hipSuccess;
-    hipLaunchKernelGGL(k_span, dim3(groups), dim3(256), 0, s, p);
+hipError_t launch_span(const FrameParams &p, hipStream_t s) {
+    if (!p.big_cap || !p.ck_cap || p.v0.H == 0u) return hipSuccess;
+    const uint32_t sy = 8u - p.tile_sx;
+    const uint32_t parts = (p.v0.H + (1u << sy) + kSpanRows - 1u) / kSpanRows;
+    const uint32_t nslots = std::max<uint32_t>(1u, std::min<uint32_t>(p.big_cap, p.v0.n_tris));
+    const size_t lds = 4u * 4u * (((kSpanRows >> sy) * p.v0.ntx + 3u) / 4u);
+    hipLaunchKernelGGL(k_span, dim3(nslots * ((parts + 3u) / 4u)), dim3(256), lds, s, p, nslots, parts);
     return hipGetLastError();
 }
 

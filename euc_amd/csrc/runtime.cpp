@@ -70,6 +70,7 @@ struct euchip_ctx {
     DevArray<uint32_t> big;     // big-triangle list
     DevArray<uint4> big_rec;    // per big triangle: setup record and first checkpoint slot
     DevArray<uint4> ck;         // row checkpoints of big triangles
+    DevArray<unsigned long long> cktab;  // their row masks and slots per (tile, big triangle), zero between frames
     size_t ck_want = 0;         // checkpoint slots the next big-triangle frames should have (from the reports)
     DevArray<uint32_t> lrec;    // line records
     DevArray<uint64_t> lkey;    // per-pixel winner keys of a lines frame
@@ -576,11 +577,18 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     if ((s = ensure(ctx, ctx->big, big_cap, "big triangles", ctx->stream))) return s;
     // Row checkpoints (big-triangle frames): optional, so a pool the budget
     // cannot hold leaves the big triangles on the walk from the row start.
+    // Tiles at most 8 rows high (an 8-bit row mask per tile and big triangle)
+    // and at most 64 MB of tile entries.
     bool ck_on = false;
-    if (big_mode) {
+    const size_t mstride = (big_cap + 3) & ~(size_t)3;
+    if (big_mode && p.tile_sx >= 5 && ntiles * mstride <= ((size_t)8 << 20)) {
+        const size_t mask_had = ctx->cktab.cap;
         ck_on = ensure(ctx, ctx->big_rec, big_cap * kBigRecQuads, "big triangles", ctx->stream) == EUCHIP_OK &&
                 ensure(ctx, ctx->ck, std::max<size_t>(ctx->ck_want, kCkInitSlots), "row checkpoints",
-                       ctx->stream) == EUCHIP_OK;
+                       ctx->stream) == EUCHIP_OK &&
+                ensure(ctx, ctx->cktab, ntiles * mstride, "row masks", ctx->stream) == EUCHIP_OK;
+        if (ck_on && ctx->cktab.cap != mask_had)
+            HIP_TRY(ctx, hipMemsetAsync(ctx->cktab.ptr, 0, ctx->cktab.cap * 8u, ctx->stream));
     }
 
     for (int attempt = 0;; attempt++) {
@@ -606,6 +614,8 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
             p.big_rec = ctx->big_rec.ptr;
             p.ck = ctx->ck.ptr;
             p.ck_cap = (uint32_t)std::min<size_t>(ctx->ck.cap, kCkMaxSlots);
+            p.cktab = ctx->cktab.ptr;
+            p.mstride = (uint32_t)mstride;
         }
         p.bins = ctx->bins.ptr;
         p.v0.tile_cap = cap;
@@ -616,10 +626,7 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.heavy_cap = (uint32_t)std::min<size_t>(ctx->heavy.cap, 0xFFFFFFF0ull);
         p.frame = ctx->frame++;
         if ((s = timed(ctx, "setup", [&] { return launch_setup(p, ctx->stream, false, 0); }))) return s;
-        if (p.ck_cap) {
-            const uint32_t groups = (uint32_t)std::min<size_t>(p.big_cap, 2048);
-            if ((s = timed(ctx, "span", [&] { return launch_span(p, ctx->stream, groups); }))) return s;
-        }
+        if (p.ck_cap && (s = timed(ctx, "span", [&] { return launch_span(p, ctx->stream); }))) return s;
         if ((s = timed(ctx, "raster", [&] { return launch_raster(p, ctx->stream, false, ctx->max_groups); }))) return s;
         ctx->counts_used[npar] = 0;
         ctx->counts_used[par] = (uint32_t)ntiles;
@@ -999,6 +1006,7 @@ euchip_status euchip_ctx_destroy(euchip_ctx *ctx) {
     release(ctx, ctx->big);
     release(ctx, ctx->big_rec);
     release(ctx, ctx->ck);
+    release(ctx, ctx->cktab);
     release(ctx, ctx->lrec);
     release(ctx, ctx->lkey);
 
