@@ -117,15 +117,20 @@ constexpr uint32_t kBigCap = 4096;
 // consecutive rows touch consecutive slots): (w0, w1, w2) at xs, then
 // (xs - X0) | (xe - X0) << 16, the row's span inside the column that starts at
 // X0.  Per (tile, big triangle) one 8-byte entry of cktab (tile * mstride +
-// slot) holds, when some row of the tile has such a span, the mask of those
-// rows (bits 32-39) and the slot of the tile's row 0 plus 8 (bits 0-31, so
-// row r's slot is the entry - 8 + r): the raster pass reads its tile's
-// entries, walks the rows from their checkpoints and clears the entries for
-// the next frame.  Tiles are then at most 8 rows high.  A frame in which some
-// big triangle finds no room in the pool (FLAG_CK_MISS) walks all its big
-// triangles from the row starts.
+// slot) holds, when some row of the tile has such a span, the frame's tag
+// (bits 40-63, ck_tag below), the mask of those rows (bits
+// 32-39) and the slot of the tile's row 0 plus 8 (bits 0-31, so row r's slot
+// is that - 8 + r); an entry with another tag is stale.  The raster pass reads
+// its tile's entries and walks the tagged rows from their checkpoints.  Tiles
+// are then at most 8 rows high.  A frame in which some big triangle finds no
+// room in the pool (FLAG_CK_MISS) walks all its big triangles from the row
+// starts.
 constexpr uint32_t kSpanRows = 64;             // rows per item of the checkpoint pass (one wave)
 constexpr uint32_t kCkNone = 0x3fffffffu;      // big record: no checkpoint slots
+// The tag of frame f in cktab, 1..0xffffff; the host clears the table before
+// each frame whose f % 0xffffff is 0, so an old entry never carries the tag of
+// a later frame.
+__host__ __device__ inline uint32_t ck_tag(uint32_t frame) { return frame % 0xffffffu + 1u; }
 constexpr uint32_t kBigRecQuads = 8;
 constexpr uint32_t kCkMaxSlots = 1u << 24;     // 256 MB (below kCkNone)
 constexpr uint32_t kCkInitSlots = 1u << 21;    // 32 MB at the first big-triangle frame
@@ -210,7 +215,7 @@ struct FrameParams {
                            // dx0, dx1), (dx2, hz0, hz1, hz2)
     uint4 *ck;             // checkpoint pool, ck_cap slots; ck_cap 0: off
     uint32_t ck_cap;
-    unsigned long long *cktab;  // row masks and slots, ntiles x mstride entries (zero between frames)
+    unsigned long long *cktab;  // tagged row masks and slots, ntiles x mstride entries
     uint32_t mstride;
     TileBin *count_next;
     uint32_t zero_n;

@@ -1002,57 +1002,69 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
     __syncthreads();  // the next batch reuses sh.tri and sh.rows_incl
 }
 
-// The big triangles of `tile` from their row checkpoints, 256 at a time: the
-// tile's cktab entries (cleared here for the next frame; one coalesced 8-byte
-// load per triangle) name the rows whose span reaches into the tile and their
-// checkpoint slots; a prefix sum over the row counts spreads those rows over
-// the lanes, and each lane walks one row from its checkpoint (weights and the
-// span inside the tile's column), with the triangle's dx, hz, flags and
-// primitive from its big record (two loads beside the checkpoint's).
+// The big triangles of `tile` from their row checkpoints, 512 at a time: the
+// tile's cktab entries of this frame (coalesced 8-byte loads, two per lane,
+// those of the first 512 issued when the tile starts: e0, e1) name the rows
+// whose span reaches into the tile and their checkpoint slots; a prefix sum over the row counts spreads those rows over the lanes,
+// and each lane walks one row from its checkpoint (weights and the span inside
+// the tile's column), with the triangle's dx, hz, flags and primitive from its
+// big record (two loads beside the checkpoint's).
+struct BigTaskLds {
+    uint32_t incl[512];  // inclusive row counts
+    uint32_t slot8[512]; // checkpoint slot of the tile's row 0, + 8
+    uint8_t mask[512];   // rows of the tile with a span in it
+};
+static_assert(sizeof(BigTaskLds) <= sizeof(TriLds) * kBatch, "big-triangle tasks fit the staging space");
+
 EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, uint32_t tile, uint32_t nbig,
-                          const TileGeom &g, uint32_t mode) {
-    uint32_t *vtab = reinterpret_cast<uint32_t *>(sh.tri);  // slot of the tile's row 0, + 8
+                          const TileGeom &g, uint32_t mode, unsigned long long e0, unsigned long long e1) {
+    BigTaskLds &bt = *reinterpret_cast<BigTaskLds *>(sh.tri);
     unsigned long long *trow = p.cktab + (size_t)tile * p.mstride;
-    for (uint32_t s0 = 0; s0 < nbig; s0 += 256u) {
-        const uint32_t s = s0 + tid;
-        unsigned long long e = 0ull;
-        if (s < nbig) {
-            e = trow[s];
-            if (e) trow[s] = 0ull;
+    for (uint32_t s0 = 0; s0 < nbig; s0 += 512u) {
+        if (s0) {
+            e0 = s0 + tid < nbig ? trow[s0 + tid] : 0ull;
+            e1 = s0 + 256u + tid < nbig ? trow[s0 + 256u + tid] : 0ull;
         }
-        const uint32_t m = (uint32_t)(e >> 32) & 0xffu;
-        const uint32_t incl = block_inclusive_sum(sh, tid, __builtin_popcount(m));
-        sh.rows_incl[tid] = incl;
-        sh.owner[tid] = (uint8_t)m;
-        vtab[tid] = (uint32_t)e;
+        const uint32_t tag = ck_tag(p.frame);
+        const uint32_t m0 = (uint32_t)(e0 >> 40) == tag ? (uint32_t)(e0 >> 32) & 0xffu : 0u;
+        const uint32_t m1 = (uint32_t)(e1 >> 40) == tag ? (uint32_t)(e1 >> 32) & 0xffu : 0u;
+        const uint32_t c0 = __builtin_popcount(m0), c1 = __builtin_popcount(m1);
+        // lane t holds slots t and 256 + t: counts of slots < 256 first
+        const uint32_t i0 = block_inclusive_sum(sh, tid, c0);
+        const uint32_t t0 = sh.wave_rows[0] + sh.wave_rows[1] + sh.wave_rows[2] + sh.wave_rows[3];
         __syncthreads();
-        const uint32_t total = sh.rows_incl[255];
+        const uint32_t i1 = block_inclusive_sum(sh, tid, c1) + t0;
+        bt.incl[tid] = i0, bt.incl[256u + tid] = i1;
+        bt.mask[tid] = (uint8_t)m0, bt.mask[256u + tid] = (uint8_t)m1;
+        bt.slot8[tid] = (uint32_t)e0, bt.slot8[256u + tid] = (uint32_t)e1;
+        __syncthreads();
+        const uint32_t total = bt.incl[511];
         for (uint32_t k = tid; k < total; k += 256u) {
-            uint32_t lo = 0, hi = 255u;  // first slot whose inclusive count exceeds k
+            uint32_t lo = 0, hi = 511u;  // first slot whose inclusive count exceeds k
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (sh.rows_incl[mid] > k) hi = mid;
+                if (bt.incl[mid] > k) hi = mid;
                 else lo = mid + 1u;
             }
-            uint32_t j = k - (lo ? sh.rows_incl[lo - 1u] : 0u);
-            uint32_t bits = sh.owner[lo];
+            uint32_t j = k - (lo ? bt.incl[lo - 1u] : 0u);
+            uint32_t bits = bt.mask[lo];
             for (; j; j--) bits &= bits - 1u;  // drop the j lowest rows
             const uint32_t r = __builtin_ctz(bits);
             const uint4 *q = p.big_rec + (size_t)(s0 + lo) * kBigRecQuads;
-            const uint4 c = p.ck[(size_t)vtab[lo] - 8u + r];
-            const uint4 e6 = q[6], e7 = q[7];
+            const uint4 c = p.ck[(size_t)bt.slot8[lo] - 8u + r];
+            const uint4 x6 = q[6], x7 = q[7];
             const uint32_t xs = g.X0 + (c.w & 0xffffu), xe = g.X0 + (c.w >> 16);
             TriRec R;
-            R.dx[0] = __uint_as_float(e6.z), R.dx[1] = __uint_as_float(e6.w), R.dx[2] = __uint_as_float(e7.x);
-            R.hz[0] = __uint_as_float(e7.y), R.hz[1] = __uint_as_float(e7.z), R.hz[2] = __uint_as_float(e7.w);
-            R.flags = e6.y >> 30;
+            R.dx[0] = __uint_as_float(x6.z), R.dx[1] = __uint_as_float(x6.w), R.dx[2] = __uint_as_float(x7.x);
+            R.hz[0] = __uint_as_float(x7.y), R.hz[1] = __uint_as_float(x7.z), R.hz[2] = __uint_as_float(x7.w);
+            R.flags = x6.y >> 30;
             const uint32_t pi0 = r * (g.TW + 1u) + (xs - g.X0);
             const float w0 = __uint_as_float(c.x), w1 = __uint_as_float(c.y), w2 = __uint_as_float(c.z);
-            if (mode == 1u) walk_span<1u>(p, R, e6.x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-            else if (mode == 2u) walk_span<2u>(p, R, e6.x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-            else walk_span<0u>(p, R, e6.x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+            if (mode == 1u) walk_span<1u>(p, R, x6.x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+            else if (mode == 2u) walk_span<2u>(p, R, x6.x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+            else walk_span<0u>(p, R, x6.x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
         }
-        __syncthreads();  // the next 256 reuse the counts, masks and slots
+        __syncthreads();  // the next 512 reuse the counts, masks and slots
     }
 }
 
@@ -1151,6 +1163,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         }
         const uint32_t cap = v.tile_cap;
         const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
+        // the tile's first 512 checkpoint entries, loaded now (used after the bin)
+        unsigned long long e0 = 0ull, e1 = 0ull;
+        if (BIG && ckp) {
+            const unsigned long long *trow = p.cktab + (size_t)tile * p.mstride;
+            if (tid < nbig) e0 = trow[tid];
+            if (256u + tid < nbig) e1 = trow[256u + tid];
+        }
         __syncthreads();
         // Runs of entries, kBatch at a time: this tile's bin and the big
         // triangles (staging drops those that miss it), then (only when it
@@ -1191,7 +1210,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             }
             raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, prim, big, g, mode);
         }
-        if (BIG && ckp && nbig) big_tasks(p, sh, tid, tile, nbig, g, mode);
+        if (BIG && ckp && nbig) big_tasks(p, sh, tid, tile, nbig, g, mode, e0, e1);
         uint32_t claimed = 0u;
         if (claim && tid == 0u) claimed = per_xcd + atomicAdd(next, 1u);
         if (inside) {
@@ -1467,7 +1486,8 @@ __global__ __launch_bounds__(256) void k_span(FrameParams p, uint32_t nslots, ui
         if (b == 0u) continue;
         const uint32_t gi = i / cols, c = i - gi * cols;
         const uint32_t slot8 = ck + c * rows + ((ty0 + gi) << sy) + 8u - R.ymin;  // tile row 0's slot + 8
-        p.cktab[((size_t)(ty0 + gi) * v.ntx + col0 + c) * p.mstride + s] = ((unsigned long long)b << 32) | slot8;
+        p.cktab[((size_t)(ty0 + gi) * v.ntx + col0 + c) * p.mstride + s] =
+            ((unsigned long long)ck_tag(p.frame) << 40) | ((unsigned long long)b << 32) | slot8;
     }
 }
 

@@ -70,7 +70,7 @@ struct euchip_ctx {
     DevArray<uint32_t> big;     // big-triangle list
     DevArray<uint4> big_rec;    // per big triangle: setup record and first checkpoint slot
     DevArray<uint4> ck;         // row checkpoints of big triangles
-    DevArray<unsigned long long> cktab;  // their row masks and slots per (tile, big triangle), zero between frames
+    DevArray<unsigned long long> cktab;  // their tagged row masks and slots per (tile, big triangle)
     size_t ck_want = 0;         // checkpoint slots the next big-triangle frames should have (from the reports)
     DevArray<uint32_t> lrec;    // line records
     DevArray<uint64_t> lkey;    // per-pixel winner keys of a lines frame
@@ -625,6 +625,8 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.heavy = ctx->heavy.ptr;
         p.heavy_cap = (uint32_t)std::min<size_t>(ctx->heavy.cap, 0xFFFFFFF0ull);
         p.frame = ctx->frame++;
+        if (p.ck_cap && p.frame % 0xffffffu == 0u)  // the tags start over: no old entry may match
+            HIP_TRY(ctx, hipMemsetAsync(ctx->cktab.ptr, 0, ctx->cktab.cap * 8u, ctx->stream));
         if ((s = timed(ctx, "setup", [&] { return launch_setup(p, ctx->stream, false, 0); }))) return s;
         if (p.ck_cap && (s = timed(ctx, "span", [&] { return launch_span(p, ctx->stream); }))) return s;
         if ((s = timed(ctx, "raster", [&] { return launch_raster(p, ctx->stream, false, ctx->max_groups); }))) return s;
