@@ -1039,30 +1039,45 @@ EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, ui
         bt.slot8[tid] = (uint32_t)e0, bt.slot8[256u + tid] = (uint32_t)e1;
         __syncthreads();
         const uint32_t total = bt.incl[511];
-        for (uint32_t k = tid; k < total; k += 256u) {
-            uint32_t lo = 0, hi = 511u;  // first slot whose inclusive count exceeds k
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (bt.incl[mid] > k) hi = mid;
-                else lo = mid + 1u;
+        // two rows per lane per turn, both rows' loads issued before either walk
+        for (uint32_t k0 = 0; k0 < total; k0 += 512u) {
+            uint4 c[2], x6[2], x7[2];
+            uint32_t r[2];
+#pragma unroll
+            for (uint32_t h = 0; h < 2u; h++) {
+                const uint32_t k = k0 + h * 256u + tid;
+                r[h] = 8u;  // no row
+                if (k >= total) continue;
+                uint32_t lo = 0, hi = 511u;  // first slot whose inclusive count exceeds k
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (bt.incl[mid] > k) hi = mid;
+                    else lo = mid + 1u;
+                }
+                uint32_t j = k - (lo ? bt.incl[lo - 1u] : 0u);
+                uint32_t bits = bt.mask[lo];
+                for (; j; j--) bits &= bits - 1u;  // drop the j lowest rows
+                r[h] = __builtin_ctz(bits);
+                const uint4 *q = p.big_rec + (size_t)(s0 + lo) * kBigRecQuads;
+                c[h] = p.ck[(size_t)bt.slot8[lo] - 8u + r[h]];
+                x6[h] = q[6], x7[h] = q[7];
             }
-            uint32_t j = k - (lo ? bt.incl[lo - 1u] : 0u);
-            uint32_t bits = bt.mask[lo];
-            for (; j; j--) bits &= bits - 1u;  // drop the j lowest rows
-            const uint32_t r = __builtin_ctz(bits);
-            const uint4 *q = p.big_rec + (size_t)(s0 + lo) * kBigRecQuads;
-            const uint4 c = p.ck[(size_t)bt.slot8[lo] - 8u + r];
-            const uint4 x6 = q[6], x7 = q[7];
-            const uint32_t xs = g.X0 + (c.w & 0xffffu), xe = g.X0 + (c.w >> 16);
-            TriRec R;
-            R.dx[0] = __uint_as_float(x6.z), R.dx[1] = __uint_as_float(x6.w), R.dx[2] = __uint_as_float(x7.x);
-            R.hz[0] = __uint_as_float(x7.y), R.hz[1] = __uint_as_float(x7.z), R.hz[2] = __uint_as_float(x7.w);
-            R.flags = x6.y >> 30;
-            const uint32_t pi0 = r * (g.TW + 1u) + (xs - g.X0);
-            const float w0 = __uint_as_float(c.x), w1 = __uint_as_float(c.y), w2 = __uint_as_float(c.z);
-            if (mode == 1u) walk_span<1u>(p, R, x6.x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-            else if (mode == 2u) walk_span<2u>(p, R, x6.x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-            else walk_span<0u>(p, R, x6.x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+#pragma unroll
+            for (uint32_t h = 0; h < 2u; h++) {
+                if (r[h] >= 8u) continue;
+                const uint32_t xs = g.X0 + (c[h].w & 0xffffu), xe = g.X0 + (c[h].w >> 16);
+                TriRec R;
+                R.dx[0] = __uint_as_float(x6[h].z), R.dx[1] = __uint_as_float(x6[h].w);
+                R.dx[2] = __uint_as_float(x7[h].x);
+                R.hz[0] = __uint_as_float(x7[h].y), R.hz[1] = __uint_as_float(x7[h].z);
+                R.hz[2] = __uint_as_float(x7[h].w);
+                R.flags = x6[h].y >> 30;
+                const uint32_t pi0 = r[h] * (g.TW + 1u) + (xs - g.X0);
+                const float w0 = __uint_as_float(c[h].x), w1 = __uint_as_float(c[h].y), w2 = __uint_as_float(c[h].z);
+                if (mode == 1u) walk_span<1u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+                else if (mode == 2u) walk_span<2u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+                else walk_span<0u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+            }
         }
         __syncthreads();  // the next 512 reuse the counts, masks and slots
     }
