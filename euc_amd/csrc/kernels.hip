@@ -754,7 +754,10 @@ EUCHIP_DEV void v_data(const TriRec &R, const float (*V)[VDN<SH>::N], float X, f
 }
 
 // Deferred shading of one pixel's winning fragment (pipeline.rs:539-576).
-template <uint32_t SH, bool GS>
+// MS: the frame's MSAA level is above 0 (coarse shading); the one-sample and
+// the coarse kernels are compiled apart so the first keep the registers the
+// second needs for its four samples.
+template <uint32_t SH, bool GS, bool MS>
 EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const View &v, const TriRec &R, uint32_t px, uint32_t py,
                                 uint32_t by0, uint32_t old) {
     constexpr uint32_t NV = VDN<SH>::N;
@@ -767,7 +770,7 @@ EUCHIP_DEV uint32_t shade_pixel(const FrameParams &p, const View &v, const TriRe
     }
     float frag[4], vd[NV];
     const uint32_t L = p.msaa_level;
-    if (L == 0u) {
+    if constexpr (!MS) {
         v_data<SH>(R, V, (float)px, (float)py, vd);
         fragment_shader<SH, GS>(v, p.sampler, vd, frag);
     } else {
@@ -1084,13 +1087,15 @@ EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, ui
 }
 
 // Waves per SIMD the raster pass is compiled for (measured per shader): latency
-// hiding pays for a few spilled registers, up to a point.
-constexpr uint32_t raster_waves(uint32_t sh) {
-    return (sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 5u : sh == TEXTURED ? 7u : (sh == DEPTH_ONLY || sh == FLAT) ? 8u : 7u;
+// hiding pays for a few spilled registers, up to a point.  Without coarse
+// shading every one-request and batched kernel fits these without spills.
+constexpr uint32_t raster_waves(uint32_t sh, bool ms) {
+    return ms ? ((sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 5u : sh == TEXTURED ? 7u : (sh == DEPTH_ONLY || sh == FLAT) ? 8u : 7u)
+              : ((sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 7u : 8u);
 }
 
-template <uint32_t SH, bool BATCH, bool BIG, bool GS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(SH)))) void k_raster(FrameParams p) {
+template <uint32_t SH, bool BATCH, bool BIG, bool GS, bool MS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(SH, MS)))) void k_raster(FrameParams p) {
     __shared__ TileShared sh;
     const uint32_t tid = threadIdx.x;
     const uint32_t ntiles = p.ntiles;
@@ -1248,7 +1253,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
                     TriRec R;
                     load_rec0(p, v.tri_base + prim, R);
                     tri_sids(v, p.geometry, prim, R.flags, R.sid);
-                    store_stream(&v.pixel[pidx], shade_pixel<SH, GS>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
+                    store_stream(&v.pixel[pidx], shade_pixel<SH, GS, MS>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
                                                     v.clear_color_en ? v.clear_color : 0u));
                 } else if (v.clear_color_en) {
                     store_stream(&v.pixel[pidx], v.clear_color);
@@ -1372,12 +1377,12 @@ hipError_t launch_setup(const FrameParams &p, hipStream_t s, bool batched, uint3
 // Workgroups of a raster kernel that fit on the device at once (`cus` compute
 // units); the busy-tile loop gives every workgroup a first tile, so workgroups
 // that could only start when others finish would hold theirs back.
-template <uint32_t SH, bool BATCH, bool BIG, bool GS>
+template <uint32_t SH, bool BATCH, bool BIG, bool GS, bool MS>
 static uint32_t resident_groups(uint32_t cus) {
     static int per_cu = 0;
     if (per_cu == 0) {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_raster<SH, BATCH, BIG, GS>),
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_raster<SH, BATCH, BIG, GS, MS>),
                                                          256, 0) != hipSuccess ||
             n < 1)
             n = 8;
@@ -1386,18 +1391,24 @@ static uint32_t resident_groups(uint32_t cus) {
     return (uint32_t)per_cu * cus;
 }
 
+template <uint32_t SH, bool GS, bool MS>
+static void launch_raster_m(const FrameParams &p, hipStream_t s, bool batched, uint32_t groups, uint32_t cus) {
+    if (batched) {
+        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, true, false, GS, MS>(cus) / 8u * 8u));
+        hipLaunchKernelGGL((k_raster<SH, true, false, GS, MS>), dim3(groups), dim3(256), 0, s, p);
+    } else if (p.big_cap) {
+        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, false, true, GS, MS>(cus) / 8u * 8u));
+        hipLaunchKernelGGL((k_raster<SH, false, true, GS, MS>), dim3(groups), dim3(256), 0, s, p);
+    } else {
+        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, false, false, GS, MS>(cus) / 8u * 8u));
+        hipLaunchKernelGGL((k_raster<SH, false, false, GS, MS>), dim3(groups), dim3(256), 0, s, p);
+    }
+}
+
 template <uint32_t SH, bool GS>
 static void launch_raster_g(const FrameParams &p, hipStream_t s, bool batched, uint32_t groups, uint32_t cus) {
-    if (batched) {
-        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, true, false, GS>(cus) / 8u * 8u));
-        hipLaunchKernelGGL((k_raster<SH, true, false, GS>), dim3(groups), dim3(256), 0, s, p);
-    } else if (p.big_cap) {
-        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, false, true, GS>(cus) / 8u * 8u));
-        hipLaunchKernelGGL((k_raster<SH, false, true, GS>), dim3(groups), dim3(256), 0, s, p);
-    } else {
-        groups = std::min(groups, std::max<uint32_t>(8u, resident_groups<SH, false, false, GS>(cus) / 8u * 8u));
-        hipLaunchKernelGGL((k_raster<SH, false, false, GS>), dim3(groups), dim3(256), 0, s, p);
-    }
+    if (p.msaa_level) launch_raster_m<SH, GS, true>(p, s, batched, groups, cus);
+    else launch_raster_m<SH, GS, false>(p, s, batched, groups, cus);
 }
 
 // Textured shaders whose sampler is not their own take the kernels that read
