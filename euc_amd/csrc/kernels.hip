@@ -1,26 +1,32 @@
 // kernels.hip -- gfx950 kernels of the triangle path.
 //
-// Frame = setup -> raster (two launches), for one request (the view in the
-// kernel argument) or for a batch of requests (views in device memory, each
-// with its own target, mesh, uniforms, tiles and bins):
+// Frame = setup -> [span] -> raster, for one request (the view in the kernel
+// argument) or for a batch of requests (views in device memory, each with its
+// own target, mesh, uniforms, tiles and bins):
 //   k_setup   one lane per shaded vertex (clip position and VertexData of the
 //             fixed vertex shader) and one lane per triangle: euc's triangle
 //             setup (triangles.rs:55-171) in IEEE f32, culling, and binning of
 //             the triangle into every tile its bounding box covers.  Pairs are
 //             grouped by tile in an LDS hash so each distinct tile costs one
 //             global atomic per workgroup; ranks past a tile's bin capacity go
-//             to a shared overflow list.
+//             to overflow segments of that tile taken from a pool.  With few
+//             triangles per tile (big-triangle mode) triangles covering many
+//             tiles go to one list instead.
+//   k_span    (big-triangle mode) walks every row of every big triangle once
+//             and stores the row's weights where it enters each tile column,
+//             plus per (tile, triangle) the rows that reach into the tile.
 //   k_raster  256-thread workgroups that take tiles (256 pixels, one per
 //             thread; 32x8, 16x16 or 8x32) from per-XCD queues.  Each pixel
 //             keeps a 64-bit key in LDS that encodes the winning fragment; the
 //             depth rule becomes an order-independent reduction (atomic min/max
 //             of the key) because every fixed shader's blend replaces the old
 //             pixel: the winner is a function of the set of fragments, with
-//             primitive order as the tie-break (see init_key).  Triangles that
-//             touch few pixels of the tile are rasterised one per lane, walking
-//             their rows exactly as triangles.rs:219-303 does; larger ones are
-//             tested pixel-parallel.  The winner is then shaded once and colour
+//             primitive order as the tie-break (see init_key).  The (triangle,
+//             row) pairs of a tile are spread over the lanes, each walking its
+//             row exactly as triangles.rs:219-303 does (big triangles from
+//             their checkpoints).  The winner is then shaded once and colour
 //             and depth are written once per pixel.
+//   k_line_*  the lines rasteriser (lines.rs:15-121) for one request.
 #include "internal.hpp"
 #include "shade.hpp"
 
