@@ -1095,13 +1095,14 @@ EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, ui
 // Waves per SIMD the raster pass is compiled for (measured per shader): latency
 // hiding pays for a few spilled registers, up to a point.  Without coarse
 // shading every one-request and batched kernel fits these without spills.
-constexpr uint32_t raster_waves(uint32_t sh, bool ms) {
+// The big-triangle kernels hold two row tasks per lane and take 7.
+constexpr uint32_t raster_waves(uint32_t sh, bool ms, bool big) {
     return ms ? ((sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 5u : sh == TEXTURED ? 7u : (sh == DEPTH_ONLY || sh == FLAT) ? 8u : 7u)
-              : ((sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 7u : 8u);
+              : ((sh == PHONG_SHADOW || sh == TEX_SHADOW || big) ? 7u : 8u);
 }
 
 template <uint32_t SH, bool BATCH, bool BIG, bool GS, bool MS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(SH, MS)))) void k_raster(FrameParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(SH, MS, BIG)))) void k_raster(FrameParams p) {
     __shared__ TileShared sh;
     const uint32_t tid = threadIdx.x;
     const uint32_t ntiles = p.ntiles;
