@@ -921,6 +921,7 @@ struct TileShared {
     uint32_t wave_rows[4];          // row counts per wave (prefix-sum carry)
     uint32_t band[2 * kMaxTileH];   // [by0, by1) of the render_par band of each row
     uint32_t next;                  // tile taken from the dynamic queue
+    uint32_t next_tile, next_count, next_head;  // batches: that tile and its bin state
 };
 
 // Rows of bin entry `prim` inside the tile, its setup kept in sh.tri[slot].
@@ -1166,6 +1167,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     // (the atomic's latency hides behind the resolve) and leaves it in sh.next.
     // Ends with a barrier (LDS is reused by the next tile).
     auto run_tile = [&](uint32_t tile, uint32_t cnt, uint32_t head, bool claim) {
+        // Batches (hundreds of tiles per workgroup) take the next queue index
+        // when the tile starts and load that tile's list entry and bin state
+        // before the resolve, so the next tile starts with no load to wait for.
+        uint32_t claimed = 0u;
+        if (BATCH && claim && tid == 0u) claimed = per_xcd + atomicAdd(next, 1u);
         uint32_t r = 0;
         if constexpr (BATCH) r = __builtin_amdgcn_readfirstlane(p.tile_req[tile]);
         const View &v = view_of<BATCH>(p, r);
@@ -1238,8 +1244,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, prim, big, g, mode);
         }
         if (BIG && ckp && nbig) big_tasks(p, sh, tid, tile, nbig, g, mode, e0, e1);
-        uint32_t claimed = 0u;
-        if (claim && tid == 0u) claimed = per_xcd + atomicAdd(next, 1u);
+        uint32_t ntile = 0u;
+        TileBin ntb{0u, 0u};
+        if (BATCH && claim && tid == 0u && claimed < nbusy) {
+            ntile = list[claimed];
+            ntb = p.tile_count[ntile];
+        }
+        if (!BATCH && claim && tid == 0u) claimed = per_xcd + atomicAdd(next, 1u);
         if (inside) {
             const uint64_t kk = sh.key[lp];
             bool has;
@@ -1271,7 +1282,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
                 else if (v.clear_depth_en) store_stream(&v.depth[pidx], v.clear_depth);
             }
         }
-        if (claim && tid == 0u) sh.next = claimed;
+        if (claim && tid == 0u) {
+            sh.next = claimed;
+            if constexpr (BATCH) sh.next_tile = ntile, sh.next_count = ntb.count, sh.next_head = ntb.head;
+        }
         __syncthreads();
     };
 
@@ -1300,10 +1314,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             uint32_t b0, b1;
             xcd_range(ntiles, xcd, b0, b1);
             tile = b0 + k;
+        } else if (BATCH && k != w) {
+            tile = __builtin_amdgcn_readfirstlane(sh.next_tile);  // loaded during the previous tile
         } else {
             tile = k == w ? first : __builtin_amdgcn_readfirstlane(list[k]);
         }
-        const TileBin tb = k == w && !nbig ? first_bin : p.tile_count[tile];
+        TileBin tb = k == w && !nbig ? first_bin : TileBin{0u, 0u};
+        if (BATCH && k != w) tb = TileBin{sh.next_count, sh.next_head};
+        else if (k != w || nbig) tb = p.tile_count[tile];
         const uint32_t cnt = __builtin_amdgcn_readfirstlane(tb.count);
         if (!(p.heavy_cap && cnt >= kHeavyTile)) {
             run_tile(tile, cnt, __builtin_amdgcn_readfirstlane(tb.head), true);
