@@ -1,0 +1,121 @@
+"""Big-triangle mode on the GPU against the oracle, bit-exact: meshes with no more
+triangles than the target has tiles, whose triangles cover many tiles each.
+
+Their rows are walked once by the checkpoint pass and resumed in every tile
+(triangles.rs:257-301 additions, unchanged); these cases force the other paths
+that mode has:
+  * a first frame whose big triangles need more checkpoint slots than a fresh
+    context holds (they then walk every row from its start), and the second frame
+    of the same context, which gets the pool that frame asked for;
+  * every tile shape the context takes for big-triangle frames: 64x4, 128x2 and
+    256x1 tiles use the checkpoints with 4-, 2- and 1-row masks, 8x32 and 16x16
+    tiles (taller than 8 rows) walk from the row starts;
+  * every depth mode, both band policies and MSAA through the checkpoint path.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from euc_amd import scenes as S
+from euc_amd.pipeline import (AaMode, BandPolicy, CoordinateMode, CullMode, DepthMode, PipelineDesc, PixelMode,
+                              Shader, Xform)
+from tests.parity import assert_bit_exact, run_pair
+
+pytestmark = pytest.mark.gpu
+
+CK_INIT_SLOTS = 1 << 21  # checkpoint slots of a fresh context
+
+
+def corner_triangles(seed, n):
+    """n triangles with one vertex near each of three screen corners (boxes that
+    cover most of the target), z in [0, 1], w = 1, one flat colour each."""
+    rng = np.random.default_rng(seed)
+    corners = np.array([[-1.0, -1.0], [1.0, -1.0], [1.0, 1.0], [-1.0, 1.0]], np.float64)
+    pos = np.empty((3 * n, 4), np.float32)
+    for t in range(n):
+        pick = rng.permutation(4)[:3]
+        for i, c in enumerate(pick):
+            pos[3 * t + i, :2] = corners[c] * rng.uniform(0.85, 1.15, 2)
+        pos[3 * t:3 * t + 3, 2] = rng.uniform(0.0, 1.0, 3)
+    pos[:, 3] = 1.0
+    rgba8 = np.repeat(rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32) | np.uint32(0xFF000000), 3)
+    return S.Mesh(pos=pos, count=3 * n, rgba8=rgba8)
+
+
+def checkpoint_slots(mesh, w, h, sx=5):
+    """Slots the checkpoint pass asks for: rows x tile columns of every triangle's
+    clamped box (as triangle setup computes it)."""
+    p = mesh.pos.reshape(-1, 3, 4).astype(np.float64)
+    xs = (p[:, :, 0] / p[:, :, 3] * 0.5 + 0.5) * w
+    ys = (p[:, :, 1] / p[:, :, 3] * -0.5 + 0.5) * h
+    xmin = np.clip(np.floor(xs.min(1)), 0, w).astype(np.int64)
+    xmax = np.clip(np.floor(xs.max(1) + 1), 0, w).astype(np.int64)
+    ymin = np.clip(np.floor(ys.min(1)), 0, h).astype(np.int64)
+    ymax = np.clip(np.floor(ys.max(1) + 1), 0, h).astype(np.int64)
+    cols = ((xmax - 1) >> sx) - (xmin >> sx) + 1
+    return int(((ymax - ymin) * cols).sum())
+
+
+def flat(dm=DepthMode.LESS_WRITE, bands=BandPolicy.PAR, aa=AaMode.NONE):
+    return PipelineDesc.make(Shader.FLAT, Xform.CLIP, PixelMode.WRITE, dm, CoordinateMode.VULKAN, CullMode.NONE, aa,
+                             bands)
+
+
+def test_checkpoint_pool_too_small_then_grown():
+    from euc_amd.renderer import Context
+    w = h = 2048
+    mesh = corner_triangles(7, 28)
+    assert checkpoint_slots(mesh, w, h) > CK_INIT_SLOTS + CK_INIT_SLOTS // 4  # the first frame cannot hold them
+    ctx = Context(0)
+    try:
+        for frame in range(2):  # the pool grows from the first frame's report
+            r = run_pair(ctx, flat(), S.uniforms(), mesh, w, h, color_init=0xFF000000)
+            assert r["frags"] > 10 ** 6
+            assert_bit_exact(r, f"2048^2 corner triangles, frame {frame}")
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("sx", [3, 4, 6, 7, 8])
+def test_big_tile_shapes(sx):
+    from euc_amd.renderer import Context
+    plain = S.plain_scene()
+    old = os.environ.get("EUCHIP_BIG_TILE_SX")
+    os.environ["EUCHIP_BIG_TILE_SX"] = str(sx)
+    try:
+        ctx = Context(0)
+    finally:
+        if old is None:
+            del os.environ["EUCHIP_BIG_TILE_SX"]
+        else:
+            os.environ["EUCHIP_BIG_TILE_SX"] = old
+    try:
+        for bands in (BandPolicy.PAR, BandPolicy.SEQ):
+            r = run_pair(ctx, flat(bands=bands), plain.uniforms, plain.mesh, 640, 480, color_init=0x11223344)
+            assert r["frags"] > 10 ** 5
+            assert_bit_exact(r, f"tiles {1 << sx}x{256 >> sx} {bands.name}")
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("dm", [DepthMode.LESS_WRITE, DepthMode.GREATER_WRITE, DepthMode.LESS_PASS, DepthMode.NONE,
+                                DepthMode(2, True)])
+def test_checkpoints_depth_modes(gpu_ctx, dm):
+    mesh = corner_triangles(11, 40)
+    init = 0.5 if dm.test in (2, 3) else 1.0
+    r = run_pair(gpu_ctx, flat(dm=dm), S.uniforms(), mesh, 700, 500, depth_init=init)
+    assert r["frags"] > 10 ** 5 or dm.test == 2
+    assert_bit_exact(r, f"corner triangles {dm}")
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_checkpoints_msaa(gpu_ctx, level):
+    mesh = corner_triangles(13, 30)
+    d = PipelineDesc.make(Shader.COLOR, Xform.CLIP, PixelMode.WRITE, DepthMode.LESS_WRITE, CoordinateMode.VULKAN,
+                          CullMode.NONE, AaMode.Msaa(level), BandPolicy.PAR)
+    m = S.Mesh(pos=mesh.pos, count=mesh.count, col=np.random.default_rng(level).uniform(0, 1, (mesh.count, 4))
+               .astype(np.float32))
+    r = run_pair(gpu_ctx, d, S.uniforms(), m, 1600, 1000)
+    assert r["frags"] > 10 ** 5
+    assert_bit_exact(r, f"corner triangles msaa {level}")
