@@ -1251,6 +1251,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             ntb = p.tile_count[ntile];
         }
         if (!BATCH && claim && tid == 0u) claimed = per_xcd + atomicAdd(next, 1u);
+        bool shade = false;  // this lane's pixel has a winner to shade
         if (inside) {
             const uint64_t kk = sh.key[lp];
             bool has;
@@ -1266,20 +1267,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
                 const uint32_t o = (uint32_t)(kk >> 32);
                 z = zunord(mode == 1u ? o : ~o, (uint32_t)kk & 1u);
             }
-            if (v.pixel) {
-                if (p.pixel_write && has && SH != DEPTH_ONLY) {
-                    TriRec R;
-                    load_rec0(p, v.tri_base + prim, R);
-                    tri_sids(v, p.geometry, prim, R.flags, R.sid);
-                    store_stream(&v.pixel[pidx], shade_pixel<SH, GS, MS>(p, v, R, px, py, sh.band[2 * (tid >> sx)],
-                                                    v.clear_color_en ? v.clear_color : 0u));
-                } else if (v.clear_color_en) {
-                    store_stream(&v.pixel[pidx], v.clear_color);
-                }
-            }
+            shade = v.pixel && p.pixel_write && has && SH != DEPTH_ONLY;
+            if (v.pixel && !shade && v.clear_color_en) store_stream(&v.pixel[pidx], v.clear_color);
             if (v.depth) {
                 if (has && p.depth_write) store_stream(&v.depth[pidx], z);
                 else if (v.clear_depth_en) store_stream(&v.depth[pidx], v.clear_depth);
+            }
+        }
+        // The pixels to shade, packed onto the first lanes in tile order: a wave
+        // with none of them skips the shader.
+        if constexpr (SH != DEPTH_ONLY) {
+            const uint64_t bal = __ballot(shade);
+            if ((tid & 63u) == 0u) sh.wave_rows[tid >> 6] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            uint32_t at = 0, total = 0;
+#pragma unroll
+            for (uint32_t w4 = 0; w4 < 4u; w4++) {
+                const uint32_t c = sh.wave_rows[w4];
+                at += w4 < (tid >> 6) ? c : 0u;
+                total += c;
+            }
+            if (shade) sh.owner[at + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u))] = (uint8_t)tid;
+            __syncthreads();
+            if (tid < total) {
+                const uint32_t t = sh.owner[tid];
+                const uint32_t tx2 = g.X0 + (t & (g.TW - 1u)), ty2 = g.Y0 + (t >> sx);
+                const uint64_t kk = sh.key[t + (t >> sx)];
+                const uint32_t prim = mode == 0u ? (uint32_t)(kk >> 32) - 1u : ((uint32_t)kk >> 1) - 1u;
+                TriRec R;
+                load_rec0(p, v.tri_base + prim, R);
+                tri_sids(v, p.geometry, prim, R.flags, R.sid);
+                store_stream(&v.pixel[(size_t)ty2 * v.W + tx2],
+                             shade_pixel<SH, GS, MS>(p, v, R, tx2, ty2, sh.band[2 * (t >> sx)],
+                                                     v.clear_color_en ? v.clear_color : 0u));
             }
         }
         if (claim && tid == 0u) {
