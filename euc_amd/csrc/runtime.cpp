@@ -577,11 +577,13 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     if ((s = ensure(ctx, ctx->big, big_cap, "big triangles", ctx->stream))) return s;
     // Row checkpoints (big-triangle frames): optional, so a pool the budget
     // cannot hold leaves the big triangles on the walk from the row start.
-    // Tiles at most 8 rows high (an 8-bit row mask per tile and big triangle)
-    // and at most 64 MB of tile entries.
+    // Tiles at most 8 rows high (an 8-bit row mask per tile and big triangle),
+    // at most 64 MB of tile entries, and the checkpoint pass's LDS mask tables
+    // (4 waves x 64 rows x tile columns bytes) within 64 KB.
     bool ck_on = false;
     const size_t mstride = (big_cap + 3) & ~(size_t)3;
-    if (big_mode && p.tile_sx >= 5 && ntiles * mstride <= ((size_t)8 << 20)) {
+    if (big_mode && p.tile_sx >= 5 && ntiles * mstride <= ((size_t)8 << 20) &&
+        4u * (kSpanRows >> (8u - p.tile_sx)) * (size_t)p.v0.ntx <= ((size_t)64 << 10)) {
         const size_t mask_had = ctx->cktab.cap;
         ck_on = ensure(ctx, ctx->big_rec, big_cap * kBigRecQuads, "big triangles", ctx->stream) == EUCHIP_OK &&
                 ensure(ctx, ctx->ck, std::max<size_t>(ctx->ck_want, kCkInitSlots), "row checkpoints",
