@@ -111,6 +111,7 @@ SIGNATURES = {
     "euchip_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "euchip_batch_destroy": (C.c_int, [C.c_void_p]),
     "euchip_batch_set": (C.c_int, [C.c_void_p, C.POINTER(PipelineDescC), C.POINTER(RequestC), C.c_uint32]),
+    "euchip_batch_set_multi": (C.c_int, [C.c_void_p, C.POINTER(PipelineDescC), C.POINTER(RequestC), C.c_uint32]),
     "euchip_batch_set_tile_width": (C.c_int, [C.c_void_p, C.c_uint32]),
     "euchip_batch_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
     "euchip_batch_launch": (C.c_int, [C.c_void_p, C.c_int]),

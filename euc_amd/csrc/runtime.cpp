@@ -154,6 +154,9 @@ struct euchip_batch {
     hipGraphExec_t exec = nullptr;
     hipEvent_t done = nullptr;
     bool launched = false;
+    // euchip_batch_set_multi: one part per distinct desc (this batch's own
+    // requests are then empty)
+    std::vector<euchip_batch *> parts;
 };
 
 namespace {
@@ -1312,8 +1315,28 @@ euchip_status euchip_batch_create(euchip_ctx *ctx, euchip_batch **out) {
     return EUCHIP_OK;
 }
 
+// Destroy the parts of a multi-desc batch.
+void drop_parts(euchip_batch *b) {
+    for (euchip_batch *q : b->parts) euchip_batch_destroy(q);
+    b->parts.clear();
+}
+
+// Two descs that render identically in one launch (every field equal).
+bool same_desc(const euchip_pipeline_desc &a, const euchip_pipeline_desc &b) {
+    return a.shader == b.shader && a.xform == b.xform && a.pixel_write == b.pixel_write &&
+           a.depth_test == b.depth_test && a.depth_write == b.depth_write && a.y_up == b.y_up &&
+           a.z_clip == b.z_clip && (!a.z_clip || (a.z_min == b.z_min && a.z_max == b.z_max)) && a.cull == b.cull &&
+           a.aa_msaa == b.aa_msaa && a.aa_level == b.aa_level && a.band_policy == b.band_policy &&
+           a.threads == b.threads && a.clear_color_enable == b.clear_color_enable &&
+           a.clear_color == b.clear_color && a.clear_depth_enable == b.clear_depth_enable &&
+           (a.clear_depth == b.clear_depth || (a.clear_depth != a.clear_depth && b.clear_depth != b.clear_depth)) &&
+           a.texture == b.texture && a.shadow == b.shadow && a.pixel_format == b.pixel_format &&
+           a.sampler == b.sampler && a.geometry == b.geometry && a.primitives == b.primitives;
+}
+
 euchip_status euchip_batch_destroy(euchip_batch *b) {
     if (!b) return EUCHIP_INVALID_ARG;
+    drop_parts(b);
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->stream);
     free_batch_memory(b);
@@ -1332,6 +1355,7 @@ euchip_status euchip_batch_set(euchip_batch *b, const euchip_pipeline_desc *desc
     if (!s) s = check_triangles(ctx, desc);
     if (s) return s;
     HIP_TRY(ctx, hipStreamSynchronize(b->stream));
+    drop_parts(b);
     std::vector<Plan> plans(n);
     std::vector<std::array<float, kUniforms>> uni(n);
     std::vector<uint32_t> active;
@@ -1360,8 +1384,59 @@ euchip_status euchip_batch_set(euchip_batch *b, const euchip_pipeline_desc *desc
     return EUCHIP_OK;
 }
 
+euchip_status euchip_batch_set_multi(euchip_batch *b, const euchip_pipeline_desc *descs, const euchip_request *reqs,
+                                     uint32_t n) {
+    if (!b || (n && (!descs || !reqs))) return fail(b ? b->ctx : nullptr, EUCHIP_INVALID_ARG, "null argument");
+    euchip_ctx *ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(ctx, hipStreamSynchronize(b->stream));
+    drop_parts(b);
+    // this batch's own requests become empty; the parts replace them
+    b->reqs.clear();
+    b->uniforms.clear();
+    b->plans.clear();
+    b->active.clear();
+    b->tables_dirty = true;
+    b->launched = false;
+    drop_graph(b);
+    euchip_status s = EUCHIP_OK;
+    std::vector<euchip_pipeline_desc> kinds;  // distinct descs, in order of first use
+    std::vector<std::vector<uint32_t>> members;
+    for (uint32_t i = 0; i < n; i++) {
+        size_t g = 0;
+        while (g < kinds.size() && !same_desc(kinds[g], descs[i])) g++;
+        if (g == kinds.size()) kinds.push_back(descs[i]), members.emplace_back();
+        members[g].push_back(i);
+    }
+    for (size_t g = 0; g < kinds.size(); g++) {
+        euchip_batch *q = nullptr;
+        if ((s = euchip_batch_create(ctx, &q))) break;
+        q->tile_sx = b->tile_sx;
+        b->parts.push_back(q);
+        std::vector<euchip_request> rs;
+        for (uint32_t i : members[g]) rs.push_back(reqs[i]);
+        if ((s = euchip_batch_set(q, &kinds[g], rs.data(), (uint32_t)rs.size()))) {
+            // name the request by its index in this call
+            const std::string msg = ctx->last_error;
+            const size_t at = msg.find("request ");
+            if (at == 0) {
+                const uint32_t k = (uint32_t)std::strtoul(msg.c_str() + 8, nullptr, 10);
+                if (k < members[g].size())
+                    ctx->last_error = "request " + std::to_string(members[g][k]) + msg.substr(msg.find(':'));
+            }
+            break;
+        }
+    }
+    if (s) drop_parts(b);
+    return s;
+}
+
 euchip_status euchip_batch_set_tile_width(euchip_batch *b, uint32_t tile_w) {
     if (!b) return EUCHIP_INVALID_ARG;
+    for (euchip_batch *q : b->parts) {
+        const euchip_status s = euchip_batch_set_tile_width(q, tile_w);
+        if (s) return s;
+    }
     const uint32_t sx = tile_w == 8 ? 3u : tile_w == 16 ? 4u : tile_w == 32 ? 5u : 0u;
     if (!sx) return fail(b->ctx, EUCHIP_INVALID_ARG, "tile width %u (8, 16 or 32)", tile_w);
     if (sx != b->tile_sx) {
@@ -1376,17 +1451,32 @@ euchip_status euchip_batch_set_tile_width(euchip_batch *b, uint32_t tile_w) {
 euchip_status euchip_batch_launch(euchip_batch *b, int use_graph) {
     if (!b) return EUCHIP_INVALID_ARG;
     (void)hipSetDevice(b->ctx->device);
-    return launch_batch(b, use_graph != 0);
+    for (euchip_batch *q : b->parts) {
+        const euchip_status s = launch_batch(q, use_graph != 0);
+        if (s) return s;
+    }
+    return b->parts.empty() ? launch_batch(b, use_graph != 0) : EUCHIP_OK;
 }
 
 euchip_status euchip_batch_wait(euchip_batch *b) {
     if (!b) return EUCHIP_INVALID_ARG;
     (void)hipSetDevice(b->ctx->device);
-    return wait_batch(b);
+    if (b->parts.empty()) return wait_batch(b);
+    euchip_status s = EUCHIP_OK;
+    for (euchip_batch *q : b->parts) {
+        const euchip_status t = wait_batch(q);
+        if (!s) s = t;
+    }
+    return s;
 }
 
 euchip_status euchip_batch_query(euchip_batch *b) {
     if (!b) return EUCHIP_INVALID_ARG;
+    for (euchip_batch *q : b->parts) {
+        const euchip_status s = euchip_batch_query(q);
+        if (s) return s;
+    }
+    if (!b->parts.empty()) return EUCHIP_OK;
     const hipError_t e = hipEventQuery(b->done);
     if (e == hipSuccess) return EUCHIP_OK;
     if (e == hipErrorNotReady) return EUCHIP_NOT_READY;
@@ -1395,6 +1485,11 @@ euchip_status euchip_batch_query(euchip_batch *b) {
 
 euchip_status euchip_batch_autotune(euchip_batch *b, uint32_t reps, uint32_t *best_tile_w, float *times_ms) {
     if (!b) return EUCHIP_INVALID_ARG;
+    for (size_t i = 0; i < b->parts.size(); i++) {
+        const euchip_status s = euchip_batch_autotune(b->parts[i], reps, i ? nullptr : best_tile_w, i ? nullptr : times_ms);
+        if (s) return s;
+    }
+    if (!b->parts.empty()) return EUCHIP_OK;
     euchip_ctx *ctx = b->ctx;
     (void)hipSetDevice(ctx->device);
     reps = std::max<uint32_t>(reps, 1u);
@@ -1434,11 +1529,16 @@ euchip_status euchip_batch_autotune(euchip_batch *b, uint32_t reps, uint32_t *be
     return euchip_batch_set_tile_width(b, best_w);
 }
 
-void *euchip_batch_stream(euchip_batch *b) { return b ? (void *)b->stream : nullptr; }
+void *euchip_batch_stream(euchip_batch *b) {
+    if (!b) return nullptr;
+    return (void *)(b->parts.empty() ? b->stream : b->parts[0]->stream);
+}
 
 size_t euchip_batch_scratch_bytes(const euchip_batch *b) {
     if (!b) return 0;
-    return b->tables.cap * 16 + b->rec0.cap * 16 +
+    size_t parts = 0;
+    for (const euchip_batch *q : b->parts) parts += euchip_batch_scratch_bytes(q);
+    return parts + b->tables.cap * 16 + b->rec0.cap * 16 +
            b->rec1.cap * 16 +
            b->vary.cap * 4 + b->tile_count.cap * sizeof(TileBin) + b->busy.cap * 4 + b->bins.cap * 4 + b->ctrbuf.cap * 4 +
            b->pool.cap * 4;

@@ -181,11 +181,21 @@ class Batch:
         self.requests = []
         ctx._children.add(self)
 
-    def set(self, desc: PipelineDesc, reqs, texture=None, shadow=None, clear_color=None, clear_depth=None):
-        d = desc_c(desc, texture, shadow, clear_color, clear_depth)
+    def set(self, desc, reqs, texture=None, shadow=None, clear_color=None, clear_depth=None):
+        """Replace the requests.  `desc` is one PipelineDesc for all of them, or a
+        list of one per request (requests of equal descs still share a launch;
+        each distinct desc runs on a stream of its own)."""
         arr = _requests_c(reqs)
         self.requests = list(reqs)  # keeps uniforms and targets alive
-        _check(self.ctx.handle, _lib.lib().euchip_batch_set(self.handle, C.byref(d), arr, len(reqs)))
+        if isinstance(desc, PipelineDesc):
+            d = desc_c(desc, texture, shadow, clear_color, clear_depth)
+            _check(self.ctx.handle, _lib.lib().euchip_batch_set(self.handle, C.byref(d), arr, len(reqs)))
+            return self
+        descs = list(desc)
+        if len(descs) != len(reqs):
+            raise ValueError(f"{len(descs)} descs for {len(reqs)} requests")
+        ds = (PipelineDescC * max(len(descs), 1))(*[desc_c(x, texture, shadow, clear_color, clear_depth) for x in descs])
+        _check(self.ctx.handle, _lib.lib().euchip_batch_set_multi(self.handle, ds, arr, len(reqs)))
         return self
 
     def set_tile_width(self, w: int):

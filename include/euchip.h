@@ -245,6 +245,16 @@ euchip_status euchip_batch_destroy(euchip_batch *b);
  * EUCHIP_BAND_ZERO, EUCHIP_INVALID_ARG) naming its index in last_error. */
 euchip_status euchip_batch_set(euchip_batch *b, const euchip_pipeline_desc *desc, const euchip_request *reqs,
                                uint32_t n);
+/* Replace the requests, request i rendered with descs[i] (the per-request desc
+ * array SURVEY's batch entry point names; no counterpart in euc).  Requests of
+ * identical descs share one setup and one raster launch; each distinct desc
+ * runs as a part of the batch on a stream of its own, so the parts overlap.
+ * launch / wait / query / set_tile_width / autotune act on every part
+ * (autotune tunes each part; best_tile_w and times_ms report the first);
+ * euchip_batch_stream returns the first part's stream.  Errors as
+ * euchip_batch_set, naming the request's index. */
+euchip_status euchip_batch_set_multi(euchip_batch *b, const euchip_pipeline_desc *descs, const euchip_request *reqs,
+                                     uint32_t n);
 /* Raster tile shape: tile_w x (256 / tile_w) pixels, tile_w 8, 16 or 32. */
 euchip_status euchip_batch_set_tile_width(euchip_batch *b, uint32_t tile_w);
 /* Launch the batch `reps` times at each tile width, keep the fastest.

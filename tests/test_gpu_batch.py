@@ -211,3 +211,78 @@ def test_budget_and_accounting(gpu_ctx):
     est = estimate_bytes(icon_desc(), [Request(m, gpu_ctx.color_target(64, 64), gpu_ctx.depth_target(64, 64),
                                                sc.uniforms)])
     assert est > 0
+
+
+def mixed_descs():
+    """Five pipelines, interleaved: shaders, depth modes, culling, MSAA and bands
+    differ between them."""
+    return [icon_desc(),
+            icon_desc(shader=Shader.GOURAUD),
+            icon_desc(depth=DepthMode.LESS_PASS),
+            icon_desc(band=BandPolicy.PAR, aa=AaMode.Msaa(1)),
+            PipelineDesc.make(Shader.PHONG, Xform.MVP, depth_mode=DepthMode.LESS_WRITE, cull=CullMode.NONE)]
+
+
+def test_batch_per_request_descs(gpu_ctx):
+    kinds = mixed_descs()
+    scenes, meshes = make_requests(gpu_ctx, range(0x7A0, 0x7AC), max_tris=6000)
+    descs = [kinds[(3 * i) % len(kinds)] for i in range(len(scenes))]
+    ref = []
+    for d, sc, m, (c, dt) in zip(descs, scenes, meshes, targets(gpu_ctx, scenes)):
+        gpu_ctx.render(d, sc.uniforms, m, c, dt, clear_color=0xFF202020, clear_depth=1.0)
+        ref.append((c.download(), dt.download()))
+    tg = targets(gpu_ctx, scenes)
+    reqs = [Request(m, c, d, sc.uniforms) for sc, m, (c, d) in zip(scenes, meshes, tg)]
+    b = gpu_ctx.batch().set(descs, reqs, clear_color=0xFF202020, clear_depth=1.0)
+    for tile_w, graph in ((32, False), (8, True), (16, True)):
+        b.set_tile_width(tile_w)
+        for (c, d), sc in zip(tg, scenes):
+            c.upload(np.full((sc.height, sc.width), GARBAGE_C, np.uint32))
+            d.upload(np.full((sc.height, sc.width), GARBAGE_D, np.float32))
+        b.run(graph=graph)
+        for i, ((c, d), (rc, rd)) in enumerate(zip(tg, ref)):
+            assert_same(c.download(), d.download(), rc, rd, f"mixed request {i} tile {tile_w} graph {graph}")
+    w, _ = b.autotune(reps=1)
+    assert w in (8, 16, 32)
+    # back to one desc for all: the parts go, the batch renders as a plain one
+    b.set(kinds[0], reqs, clear_color=0xFF202020, clear_depth=1.0)
+    b.run()
+    for i, (c, d) in enumerate(tg):
+        if descs[i] == kinds[0]:
+            assert_same(c.download(), d.download(), ref[i][0], ref[i][1], f"single-desc request {i}")
+    b.close()
+
+
+def test_batch_per_request_descs_match_oracle(gpu_ctx):
+    kinds = mixed_descs()
+    scenes, meshes = make_requests(gpu_ctx, range(0x7C0, 0x7C5), max_tris=2500, max_side=96)
+    ref = []
+    for d, sc in zip(kinds, scenes):
+        pix = np.full((sc.height, sc.width), 0xFF202020, np.uint32)
+        dep = np.full((sc.height, sc.width), 1.0, np.float32)
+        rc, _ = O.render(d, sc.uniforms, sc.mesh, pix, dep)
+        assert rc == 0
+        ref.append((pix, dep))
+    tg = targets(gpu_ctx, scenes)
+    reqs = [Request(m, c, d, sc.uniforms) for sc, m, (c, d) in zip(scenes, meshes, tg)]
+    b = gpu_ctx.batch().set(kinds, reqs, clear_color=0xFF202020, clear_depth=1.0)
+    b.run()
+    for i, ((c, d), (rc, rd)) in enumerate(zip(tg, ref)):
+        assert_same(c.download(), d.download(), rc, rd, f"mixed request {i} vs oracle")
+    b.close()
+
+
+def test_batch_per_request_desc_errors(gpu_ctx):
+    sc = S.icon_request(0x610, max_tris=500)
+    m = gpu_ctx.mesh(sc.mesh)
+    ok = [Request(m, gpu_ctx.color_target(32, 32), gpu_ctx.depth_target(32, 32), sc.uniforms) for _ in range(3)]
+    bad = Request(m, gpu_ctx.color_target(32, 32), gpu_ctx.depth_target(48, 32), sc.uniforms)
+    b = gpu_ctx.batch()
+    with pytest.raises(EuchipError) as e:  # the bad request is the third of its desc, the fourth of the call
+        b.set([icon_desc(), icon_desc(shader=Shader.GOURAUD), icon_desc(), icon_desc()], ok + [bad])
+    assert e.value.name == "SIZE_MISMATCH" and "request 3" in str(e.value)
+    with pytest.raises(ValueError):
+        b.set([icon_desc()], ok)
+    b.set([icon_desc(), icon_desc(shader=Shader.GOURAUD), icon_desc()], ok)
+    b.run()
+    b.close()
