@@ -839,6 +839,32 @@ template <uint32_t MODE>
 EUCHIP_DEV void walk_span(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t xs, uint32_t xe, uint32_t pi0,
                           float w0, float w1, float w2, uint64_t *key, const float *d0s) {
     const bool clip = !(R.flags & TRI_ALL_IN) && p.z_clip;
+    if constexpr (MODE != 0u) {
+        // Write modes: a NaN z never wins.  -0 + 0 = +0 gives both zeros one
+        // order value; the key's low bit keeps the sign.  Rows of triangles
+        // that need no z clip (every lane of the wave) take the loop without it.
+        const uint32_t lo0 = (prim + 1u) << 1, lo1 = lo0 | 1u;
+        unsigned long long *kp = (unsigned long long *)(key + pi0);
+        unsigned long long *const ke = kp + (xe - xs);
+        const bool any_clip = __builtin_amdgcn_ballot_w64(clip) != 0ull;
+        const float zlo = p.z_min, zhi = p.z_max;
+        for (; kp < ke; kp++) {
+            const float u2 = (w2 - w0) - w1;
+            const float z = (R.hz[0] * w0 + R.hz[1] * w1) + R.hz[2] * u2;
+            bool ok = (w0 >= 0.0f) & (w1 >= 0.0f) & (u2 >= 0.0f) & (z == z);
+            if (any_clip) ok = ok & (!clip | ((zlo <= z) & (z <= zhi)));
+            if (ok) {
+                const uint32_t u = __float_as_uint(z + 0.0f);
+                const uint32_t o = u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
+                const uint32_t lo = __float_as_uint(z) == 0x80000000u ? lo1 : lo0;
+                atomicMin(kp, ((uint64_t)(MODE == 1u ? o : ~o) << 32) | lo);
+            }
+            w0 = w0 + R.dx[0];
+            w1 = w1 + R.dx[1];
+            w2 = w2 + R.dx[2];
+        }
+        return;
+    }
     const float zlo = p.z_min, zhi = p.z_max;
     const uint32_t test = p.depth_test;
     const uint32_t pid = prim + 1u;
@@ -848,15 +874,9 @@ EUCHIP_DEV void walk_span(const FrameParams &p, const TriRec &R, uint32_t prim, 
         bool ok = (w0 >= 0.0f) & (w1 >= 0.0f) & (u2 >= 0.0f);
         ok = ok & (!clip | ((zlo <= z) & (z <= zhi)));
         const uint32_t pi = pi0 + (x - xs);
-        if constexpr (MODE == 0u) {
-            const float D0 = d0s[pi];
-            const bool pass = test == DT_LESS ? z < D0 : test == DT_EQUAL ? z == D0 : test == DT_GREATER ? z > D0 : true;
-            if (ok & pass) atomicMax((unsigned long long *)(key + pi), ((uint64_t)pid << 32) | __float_as_uint(z));
-        } else {
-            const uint32_t o = MODE == 1u ? zord(z) : ~zord(z);
-            const uint32_t nz = __float_as_uint(z) == 0x80000000u ? 1u : 0u;
-            if (ok & (z == z)) atomicMin((unsigned long long *)(key + pi), ((uint64_t)o << 32) | (pid << 1) | nz);
-        }
+        const float D0 = d0s[pi];
+        const bool pass = test == DT_LESS ? z < D0 : test == DT_EQUAL ? z == D0 : test == DT_GREATER ? z > D0 : true;
+        if (ok & pass) atomicMax((unsigned long long *)(key + pi), ((uint64_t)pid << 32) | __float_as_uint(z));
         w0 = w0 + R.dx[0];
         w1 = w1 + R.dx[1];
         w2 = w2 + R.dx[2];
@@ -1015,22 +1035,28 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
 // The big triangles of `tile` from their row checkpoints, 512 at a time: the
 // tile's cktab entries of this frame (coalesced 8-byte loads, two per lane,
 // those of the first 512 issued when the tile starts: e0, e1) name the rows
-// whose span reaches into the tile and their checkpoint slots; a prefix sum over the row counts spreads those rows over the lanes,
-// and each lane walks one row from its checkpoint (weights and the span inside
-// the tile's column), with the triangle's dx, hz, flags and primitive from its
-// big record (two loads beside the checkpoint's).
+// whose span reaches into the tile and their checkpoint slots.  A prefix sum
+// over the row counts places each entry's rows in a task table (entry, row),
+// and each lane takes table entries and walks one row from its checkpoint
+// (weights and the span inside the tile's column), with the triangle's dx, hz,
+// flags and primitive from its big record (two loads beside the checkpoint's).
+constexpr uint32_t kBigChunk = 512;  // cktab entries per round
 struct BigTaskLds {
-    uint32_t incl[512];  // inclusive row counts
-    uint32_t slot8[512]; // checkpoint slot of the tile's row 0, + 8
-    uint8_t mask[512];   // rows of the tile with a span in it
+    uint32_t slot8[kBigChunk];        // checkpoint slot of the tile's row 0, + 8
+    uint16_t task[kBigChunk * 8u];    // entry << 3 | row of the tile, in entry order
 };
 static_assert(sizeof(BigTaskLds) <= sizeof(TriLds) * kBatch, "big-triangle tasks fit the staging space");
+
+// Appends entry `e`'s rows (mask bits) to the task table from position `at`.
+EUCHIP_DEV void put_tasks(BigTaskLds &bt, uint32_t e, uint32_t mask, uint32_t at) {
+    for (; mask; mask &= mask - 1u) bt.task[at++] = (uint16_t)((e << 3) | (uint32_t)__builtin_ctz(mask));
+}
 
 EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, uint32_t tile, uint32_t nbig,
                           const TileGeom &g, uint32_t mode, unsigned long long e0, unsigned long long e1) {
     BigTaskLds &bt = *reinterpret_cast<BigTaskLds *>(sh.tri);
     unsigned long long *trow = p.cktab + (size_t)tile * p.mstride;
-    for (uint32_t s0 = 0; s0 < nbig; s0 += 512u) {
+    for (uint32_t s0 = 0; s0 < nbig; s0 += kBigChunk) {
         if (s0) {
             e0 = s0 + tid < nbig ? trow[s0 + tid] : 0ull;
             e1 = s0 + 256u + tid < nbig ? trow[s0 + 256u + tid] : 0ull;
@@ -1038,17 +1064,18 @@ EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, ui
         const uint32_t tag = ck_tag(p.frame);
         const uint32_t m0 = (uint32_t)(e0 >> 40) == tag ? (uint32_t)(e0 >> 32) & 0xffu : 0u;
         const uint32_t m1 = (uint32_t)(e1 >> 40) == tag ? (uint32_t)(e1 >> 32) & 0xffu : 0u;
+        // lane t holds entries t and 256 + t: one prefix sum over both counts,
+        // entry t's rows first, then entry 256 + t's after all of the first half
         const uint32_t c0 = __builtin_popcount(m0), c1 = __builtin_popcount(m1);
-        // lane t holds slots t and 256 + t: counts of slots < 256 first
         const uint32_t i0 = block_inclusive_sum(sh, tid, c0);
         const uint32_t t0 = sh.wave_rows[0] + sh.wave_rows[1] + sh.wave_rows[2] + sh.wave_rows[3];
         __syncthreads();
         const uint32_t i1 = block_inclusive_sum(sh, tid, c1) + t0;
-        bt.incl[tid] = i0, bt.incl[256u + tid] = i1;
-        bt.mask[tid] = (uint8_t)m0, bt.mask[256u + tid] = (uint8_t)m1;
+        const uint32_t total = t0 + sh.wave_rows[0] + sh.wave_rows[1] + sh.wave_rows[2] + sh.wave_rows[3];
         bt.slot8[tid] = (uint32_t)e0, bt.slot8[256u + tid] = (uint32_t)e1;
+        put_tasks(bt, tid, m0, i0 - c0);
+        put_tasks(bt, 256u + tid, m1, i1 - c1);
         __syncthreads();
-        const uint32_t total = bt.incl[511];
         // two rows per lane per turn, both rows' loads issued before either walk
         for (uint32_t k0 = 0; k0 < total; k0 += 512u) {
             uint4 c[2], x6[2], x7[2];
@@ -1058,16 +1085,8 @@ EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, ui
                 const uint32_t k = k0 + h * 256u + tid;
                 r[h] = 8u;  // no row
                 if (k >= total) continue;
-                uint32_t lo = 0, hi = 511u;  // first slot whose inclusive count exceeds k
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (bt.incl[mid] > k) hi = mid;
-                    else lo = mid + 1u;
-                }
-                uint32_t j = k - (lo ? bt.incl[lo - 1u] : 0u);
-                uint32_t bits = bt.mask[lo];
-                for (; j; j--) bits &= bits - 1u;  // drop the j lowest rows
-                r[h] = __builtin_ctz(bits);
+                const uint32_t tk = bt.task[k], lo = tk >> 3;
+                r[h] = tk & 7u;
                 const uint4 *q = p.big_rec + (size_t)(s0 + lo) * kBigRecQuads;
                 c[h] = p.ck[(size_t)bt.slot8[lo] - 8u + r[h]];
                 x6[h] = q[6], x7[h] = q[7];
@@ -1089,7 +1108,7 @@ EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, ui
                 else walk_span<0u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
             }
         }
-        __syncthreads();  // the next 512 reuse the counts, masks and slots
+        __syncthreads();  // the next 512 reuse the task table and slots
     }
 }
 
