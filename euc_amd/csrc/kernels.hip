@@ -814,9 +814,9 @@ struct TileGeom {
 };
 
 // The tile's pixels [xs, xe) of one row (LDS index pi0 of the first), w_hom at
-// xs given: each pixel's fragment updates its winner key.  Specialised by key
-// mode so the write modes' loops hold no LDS read (nothing then waits on the
-// atomics of earlier pixels).
+// xs given: each pixel's fragment updates its winner key.  Rows of small
+// triangles in key mode 0 (the test reads the depth before the pass); the
+// write modes take walk_span's loop.
 template <uint32_t MODE>
 EUCHIP_DEV void walk_row(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t xs, uint32_t xe, uint32_t pi0,
                          float w0, float w1, float w2, uint64_t *key, const float *d0s) {
@@ -832,9 +832,10 @@ EUCHIP_DEV void walk_row(const FrameParams &p, const TriRec &R, uint32_t prim, u
     }
 }
 
-// walk_row for the big-triangle tasks, whose rows are long: coverage, z, z
-// clip and the key of every pixel without branches (z is only used where the
-// pixel is covered), one predicated LDS atomic.
+// walk_row without branches on the coverage: coverage, z, z clip and the key
+// of every pixel (z is only used where the pixel is covered), one predicated
+// LDS atomic.  The write modes' loop holds no LDS read, so nothing waits on the
+// atomics of earlier pixels.
 template <uint32_t MODE>
 EUCHIP_DEV void walk_span(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t xs, uint32_t xe, uint32_t pi0,
                           float w0, float w1, float w2, uint64_t *key, const float *d0s) {
@@ -916,8 +917,8 @@ EUCHIP_DEV void raster_row(const FrameParams &p, const TriRec &R, uint32_t prim,
         w2 = w2 + R.dx[2];
     }
     const uint32_t pi0 = r * (g.TW + 1u) + (xs - g.X0);
-    if (mode == 1u) walk_row<1u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
-    else if (mode == 2u) walk_row<2u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
+    if (mode == 1u) walk_span<1u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
+    else if (mode == 2u) walk_span<2u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
     else walk_row<0u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
 }
 
