@@ -1116,10 +1116,11 @@ EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, ui
 // Waves per SIMD the raster pass is compiled for (measured per shader): latency
 // hiding pays for a few spilled registers, up to a point.  Without coarse
 // shading every one-request and batched kernel fits these without spills.
-// The big-triangle kernels hold two row tasks per lane and take 7.
+// The big-triangle kernels hold two row tasks per lane and take 6 (their
+// frames have few tiles, so fewer waves per SIMD than that are resident anyway).
 constexpr uint32_t raster_waves(uint32_t sh, bool ms, bool big) {
     return ms ? ((sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 5u : sh == TEXTURED ? 7u : (sh == DEPTH_ONLY || sh == FLAT) ? 8u : 7u)
-              : ((sh == PHONG_SHADOW || sh == TEX_SHADOW || big) ? 7u : 8u);
+              : big ? 6u : ((sh == PHONG_SHADOW || sh == TEX_SHADOW) ? 7u : 8u);
 }
 
 template <uint32_t SH, bool BATCH, bool BIG, bool GS, bool MS>
