@@ -1275,16 +1275,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         bool shade = false;  // this lane's pixel has a winner to shade
         if (inside) {
             const uint64_t kk = sh.key[lp];
-            bool has;
-            uint32_t prim;
+            bool has;  // a winner (its primitive is read again by the shading lanes)
             float z;
             if (mode == 0u) {
                 has = kk != 0ull;
-                prim = (uint32_t)(kk >> 32) - 1u;
                 z = __uint_as_float((uint32_t)kk);
             } else {
                 has = ((uint32_t)kk) != 0u;
-                prim = ((uint32_t)kk >> 1) - 1u;
                 const uint32_t o = (uint32_t)(kk >> 32);
                 z = zunord(mode == 1u ? o : ~o, (uint32_t)kk & 1u);
             }
