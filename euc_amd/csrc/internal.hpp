@@ -47,9 +47,7 @@ enum TriFlags : uint32_t {
 //                          large triangles), 2 words unused
 constexpr uint32_t kRec0Quads = 4, kRec1Quads = 2;
 
-// FLAG_CK_MISS: a big triangle found no room in the checkpoint pool; the
-// frame's big triangles then walk their rows from the row starts.
-enum FrameFlags : uint32_t { FLAG_BIN_OVERFLOW = 1u, FLAG_CK_MISS = 2u };
+enum FrameFlags : uint32_t { FLAG_BIN_OVERFLOW = 1u };
 
 // Device counters of one launch stream, per frame parity f & 1: setup of frame f
 // counts into the [f & 1] entries, its raster pass reads them and zeroes the
@@ -63,7 +61,6 @@ struct Counters {
     uint32_t busy[2][8][32];  // per XCD: entries of the busy-tile list
     uint32_t flags[2][32];    // FrameFlags of the frame (the pool ran out: it was dropped)
     uint32_t max_count[2][32];// largest bin count of a tile past its capacity (one-request frames)
-    unsigned long long ck[2][16];  // 16-byte slots of the checkpoint pool taken
 };
 
 // What a one-request frame's raster pass reports to host-visible memory when
@@ -74,7 +71,6 @@ struct Report {
     uint32_t flags;      // FrameFlags
     uint32_t pool_used;  // pool words the frame asked for (more than pool_cap when it ran out)
     uint32_t dropped;    // frames dropped so far on this context (running count)
-    uint32_t ck_used;    // checkpoint slots the frame asked for (saturated)
     uint32_t frame;      // the frame these counts belong to
 };
 
@@ -99,41 +95,40 @@ constexpr uint32_t kHeavyTile = 128u;
 enum BusyMap : uint32_t { BUSY_RANGES = 0, BUSY_BLOCKS = 1 };
 
 // One-request frames of meshes with no more triangles than the target has tiles
-// ("big-triangle mode"): a triangle whose box covers more tiles than this goes
-// to the big-triangle list instead of every tile's bin (while the list has
-// room), and every tile of the target scans that list (a raster kernel of its
-// own).  Binning such a triangle costs one global atomic per tile when the
-// workgroup's LDS grouping overflows.
+// ("big-triangle mode"): a triangle whose box covers more tiles than this is
+// not binned (binning it costs one global atomic per tile when the
+// workgroup's LDS grouping overflows).  With row checkpoints (below) every
+// tile scans its checkpoint entries of all triangles; without them the big
+// triangles go to the big-triangle list (while it has room) and every tile
+// walks them from the row starts.
 constexpr uint32_t kBigTiles = 64;
 constexpr uint32_t kBigCap = 4096;
 
 // Row checkpoints of big triangles.  The exact walk of a row (w += dx from the
 // row start x0, triangles.rs:257-301) reaches a tile only after every pixel
-// left of it; a pass between setup and raster walks each row of each big
-// triangle once and keeps, per tile column the row's span touches, the weights
-// at the first pixel of the span in that column.  A big triangle with `rows`
-// rows and `cols` tile columns in its box takes rows * cols 16-byte slots of
-// the checkpoint pool, column-major (slot (c - col0) * rows + r, so lanes on
-// consecutive rows touch consecutive slots): (w0, w1, w2) at xs, then
+// left of it; the span blocks of the setup launch walk each row of each big
+// triangle once and keep, per tile column the row's span touches, the weights
+// at the first pixel of the span in that column.  They set up their triangle
+// themselves (the setup blocks leave big triangles out of the bins), so the
+// two kinds of block run side by side, and every slot has a fixed place: the
+// checkpoint of triangle t, tile column c and row y is slot (t * ntx + c) * H
+// + y of the pool (one 16-byte slot per triangle, column and row of the
+// target, so the 8 rows of one tile are consecutive): (w0, w1, w2) at xs, then
 // (xs - X0) | (xe - X0) << 16, the row's span inside the column that starts at
-// X0.  Per (tile, big triangle) one 8-byte entry of cktab (tile * mstride +
-// slot) holds, when some row of the tile has such a span, the frame's tag
-// (bits 40-63, ck_tag below), the mask of those rows (bits
-// 32-39) and the slot of the tile's row 0 plus 8 (bits 0-31, so row r's slot
-// is that - 8 + r); an entry with another tag is stale.  The raster pass reads
-// its tile's entries and walks the tagged rows from their checkpoints.  Tiles
-// are then at most 8 rows high.  A frame in which some big triangle finds no
-// room in the pool (FLAG_CK_MISS) walks all its big triangles from the row
-// starts.
-constexpr uint32_t kSpanRows = 64;             // rows per item of the checkpoint pass (one wave)
-constexpr uint32_t kCkNone = 0x3fffffffu;      // big record: no checkpoint slots
+// X0.  Per (tile, triangle) one 8-byte entry of cktab (tile * mstride + t)
+// holds, when some row of the tile has such a span, the frame's tag (bits
+// 40-63, ck_tag below) and the mask of those rows (bits 32-39); an entry with
+// another tag is stale.  The raster pass reads its tile's entries and walks the
+// tagged rows from their checkpoints.  Tiles are then at most 8 rows high.
+constexpr uint32_t kSpanRows = 64;             // rows per item of the checkpoint walk (one wave)
 // The tag of frame f in cktab, 1..0xffffff; the host clears the table before
 // each frame whose f % 0xffffff is 0, so an old entry never carries the tag of
 // a later frame.
 __host__ __device__ inline uint32_t ck_tag(uint32_t frame) { return frame % 0xffffffu + 1u; }
-constexpr uint32_t kBigRecQuads = 8;
-constexpr uint32_t kCkMaxSlots = 1u << 24;     // 256 MB (below kCkNone)
-constexpr uint32_t kCkInitSlots = 1u << 21;    // 32 MB at the first big-triangle frame
+// Per triangle of a checkpoint frame (big_rec): (primitive, flags << 30, dx0,
+// dx1), (dx2, hz0, hz1, hz2), what the raster's row tasks read besides the slot.
+constexpr uint32_t kBigRecQuads = 2;
+constexpr size_t kCkMaxSlots = (size_t)1 << 26;  // 1 GiB of checkpoints at most
 
 // One render request: a target pair, a mesh, a uniform block, and where the
 // request's vertices, tiles and bins sit in the launch's scratch memory.
@@ -210,13 +205,17 @@ struct FrameParams {
     uint32_t busy_map;
     uint32_t *big;         // big triangles (primitive ids), big_cap slots; big_cap 0: off (batches)
     uint32_t big_cap;
-    uint4 *big_rec;        // with ck_cap: per big triangle kBigRecQuads quads: its setup record
-                           // (rec0, rec1), then (primitive, first checkpoint slot (~0u: none),
-                           // dx0, dx1), (dx2, hz0, hz1, hz2)
-    uint4 *ck;             // checkpoint pool, ck_cap slots; ck_cap 0: off
-    uint32_t ck_cap;
-    unsigned long long *cktab;  // tagged row masks and slots, ntiles x mstride entries
+    // Row checkpoints (ck_on): per triangle its big record (kBigRecQuads
+    // quads), the pool of n_tris x ntx x H slots and the tagged row masks
+    // (ntiles x mstride entries, mstride >= n_tris); the setup launch then has
+    // setup_blocks setup blocks, then the span blocks (span_parts items of
+    // kSpanRows rows per triangle, 4 per block).
+    uint4 *big_rec;
+    uint4 *ck;
+    uint32_t ck_on;
+    unsigned long long *cktab;
     uint32_t mstride;
+    uint32_t setup_blocks, span_parts;
     TileBin *count_next;
     uint32_t zero_n;
     uint32_t frame;        // frame parity selects the overflow counter and tile queue
@@ -262,7 +261,6 @@ hipError_t launch_setup(const FrameParams &p, hipStream_t s, bool batched, uint3
 hipError_t launch_raster(const FrameParams &p, hipStream_t s, bool batched, uint32_t max_groups);
 // Row checkpoints of the big triangles (one-request big-triangle frames with a
 // checkpoint pool), between setup and raster.
-hipError_t launch_span(const FrameParams &p, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s);
 // The lines path of one request (p.v0): vertex stage + line setup, winner keys
 // initialised, lines walked, then every pixel resolved (colour, depth, clears).

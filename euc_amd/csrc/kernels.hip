@@ -1,6 +1,6 @@
 // kernels.hip -- gfx950 kernels of the triangle path.
 //
-// Frame = setup -> [span] -> raster, for one request (the view in the kernel
+// Frame = setup -> raster, for one request (the view in the kernel
 // argument) or for a batch of requests (views in device memory, each with its
 // own target, mesh, uniforms, tiles and bins):
 //   k_setup   one lane per shaded vertex (clip position and VertexData of the
@@ -11,10 +11,11 @@
 //             global atomic per workgroup; ranks past a tile's bin capacity go
 //             to overflow segments of that tile taken from a pool.  With few
 //             triangles per tile (big-triangle mode) triangles covering many
-//             tiles go to one list instead.
-//   k_span    (big-triangle mode) walks every row of every big triangle once
-//             and stores the row's weights where it enters each tile column,
-//             plus per (tile, triangle) the rows that reach into the tile.
+//             tiles are not binned: span blocks of the same launch set each
+//             one up again, walk its rows once and store the row's weights
+//             where it enters each tile column, plus per (tile, triangle) the
+//             rows that reach into the tile (or, without checkpoints, the
+//             setup blocks put it on one list).
 //   k_raster  256-thread workgroups that take tiles (256 pixels, one per
 //             thread; 32x8, 16x16 or 8x32) from per-XCD queues.  Each pixel
 //             keeps a 64-bit key in LDS that encodes the winning fragment; the
@@ -462,6 +463,10 @@ static_assert(kHash == 512u && kPairs <= (1u << 23), "pairs pack a 9-bit hash sl
 // distinct tile costs one global atomic per workgroup; ranks inside the
 // group come from LDS.
 // ---------------------------------------------------------------------------
+// A span block of a checkpoint frame's setup launch (defined with the raster's
+// row helpers below).
+EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b);
+
 // LDS of k_setup: the workgroup's records on their way out, then the binning tables.
 union SetupShared {
     struct {
@@ -480,6 +485,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     __shared__ SetupShared sm;
     __shared__ uint32_t npl;
     __shared__ uint32_t lcnt[9], lbase[9];  // new busy tiles per list (8) and new heavy tiles (8)
+    if constexpr (!BATCH) {
+        if (blockIdx.x >= p.setup_blocks) {  // a span block of a checkpoint frame
+            span_item(p, blockIdx.x - p.setup_blocks);
+            return;
+        }
+    }
     const uint32_t tid = threadIdx.x;
     uint32_t r = 0, j = blockIdx.x;
     if constexpr (BATCH) {
@@ -501,29 +512,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
             ty0 = R.ymin >> sy;
             nx = ((R.xmax - 1u) >> sx) - tx0 + 1u;
             ny = ((R.ymax - 1u) >> sy) - ty0 + 1u;
-            if (p.big_cap && nx * ny > kBigTiles) {  // on the big-triangle list instead of the bins
-                const uint32_t k = atomicAdd(&p.ctr->big[p.frame & 1u][0], 1u);
-                if (k < p.big_cap) {
-                    p.big[k] = t;
-                    if (p.ck_cap) {  // room for its row checkpoints, if the pool has it
-                        const unsigned long long need = (unsigned long long)(R.ymax - R.ymin) * nx;
-                        const unsigned long long at = atomicAdd(&p.ctr->ck[p.frame & 1u][0], need);
-                        if (at + need > p.ck_cap) atomicOr(&p.ctr->flags[p.frame & 1u][0], FLAG_CK_MISS);
-                        uint4 q0[kRec0Quads], q1[kRec1Quads];
-                        pack_rec(R, q0, q1);
-                        uint4 *dst = p.big_rec + (size_t)k * kBigRecQuads;
-#pragma unroll
-                        for (uint32_t i = 0; i < kRec0Quads; i++) dst[i] = q0[i];
-#pragma unroll
-                        for (uint32_t i = 0; i < kRec1Quads; i++) dst[kRec0Quads + i] = q1[i];
-                        // what the raster pass's checkpoint tasks read: primitive, first
-                        // slot | flags << 30, dx, hz
-                        dst[6] = make_uint4(t, (at + need <= p.ck_cap ? (uint32_t)at : kCkNone) | (R.flags << 30),
-                                            __float_as_uint(R.dx[0]), __float_as_uint(R.dx[1]));
-                        dst[7] = make_uint4(__float_as_uint(R.dx[2]), __float_as_uint(R.hz[0]), __float_as_uint(R.hz[1]),
-                                            __float_as_uint(R.hz[2]));
+            if (nx * ny > kBigTiles) {  // not binned
+                if (p.ck_on) {
+                    nx = ny = 0u;  // the span blocks walk it
+                } else if (p.big_cap) {  // on the big-triangle list (while it has room)
+                    const uint32_t k = atomicAdd(&p.ctr->big[p.frame & 1u][0], 1u);
+                    if (k < p.big_cap) {
+                        p.big[k] = t;
+                        nx = ny = 0u;
                     }
-                    nx = ny = 0u;
                 }
             }
             uint4 q0[kRec0Quads], q1[kRec1Quads];
@@ -946,26 +943,17 @@ struct TileShared {
 };
 
 // Rows of bin entry `prim` inside the tile, its setup kept in sh.tri[slot].
-// `big` != ~0u: the entry is big triangle `big`, read from its big record
-// (primitive and checkpoints included) instead of the setup records.
 EUCHIP_DEV uint32_t stage_entry(const FrameParams &p, const View &v, TileShared &sh, uint32_t slot, uint32_t prim,
-                                uint32_t big, const TileGeom &g) {
+                                const TileGeom &g) {
     TriRec R;
-    const uint4 *bq = big != ~0u ? p.big_rec + (size_t)big * kBigRecQuads : nullptr;
-    if (bq) {
-        load_rec0_at(bq, R);
-        prim = bq[6].x;
-    } else {
-        load_rec0(p, v.tri_base + prim, R);
-    }
+    load_rec0(p, v.tri_base + prim, R);
     const uint32_t ry0 = R.ymin > g.Y0 ? R.ymin : g.Y0;
     const uint32_t ry1 = R.ymax < g.Y0 + g.TH ? R.ymax : g.Y0 + g.TH;
     const uint32_t xs = R.xmin > g.X0 ? R.xmin : g.X0;
     const uint32_t xe = R.xmax < g.X0 + g.TW ? R.xmax : g.X0 + g.TW;
     if (ry0 >= ry1 || xs >= xe) return 0u;
     if (!all_rows_boxed(R)) {
-        if (bq) load_rec1_at(bq + kRec0Quads, R);
-        else load_rec1(p, v.tri_base + prim, R);
+        load_rec1(p, v.tri_base + prim, R);
     } else {
 #pragma unroll
         for (int i = 0; i < 6; i++) R.vby[i] = 0.0f;  // boxed rows never read verts_by_y
@@ -994,8 +982,8 @@ EUCHIP_DEV uint32_t block_inclusive_sum(TileShared &sh, uint32_t tid, uint32_t v
 // lane walks much more than one row.
 template <bool OWNER, bool LONG_ROWS>
 EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh, uint32_t tid, bool valid,
-                             uint32_t prim, uint32_t big, const TileGeom &g, uint32_t mode) {
-    const uint32_t rows = valid ? stage_entry(p, v, sh, tid, prim, big, g) : 0u;
+                             uint32_t prim, const TileGeom &g, uint32_t mode) {
+    const uint32_t rows = valid ? stage_entry(p, v, sh, tid, prim, g) : 0u;
     const uint32_t incl = block_inclusive_sum(sh, tid, rows);
     const uint32_t excl = incl - rows;
     sh.rows_incl[tid] = incl;
@@ -1033,83 +1021,92 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
     __syncthreads();  // the next batch reuses sh.tri and sh.rows_incl
 }
 
-// The big triangles of `tile` from their row checkpoints, 512 at a time: the
-// tile's cktab entries of this frame (coalesced 8-byte loads, two per lane,
-// those of the first 512 issued when the tile starts: e0, e1) name the rows
-// whose span reaches into the tile and their checkpoint slots.  A prefix sum
-// over the row counts places each entry's rows in a task table (entry, row),
-// and each lane takes table entries and walks one row from its checkpoint
-// (weights and the span inside the tile's column), with the triangle's dx, hz,
-// flags and primitive from its big record (two loads beside the checkpoint's).
-constexpr uint32_t kBigChunk = 512;  // cktab entries per round
+// The big triangles of `tile` from their row checkpoints, 1024 triangles at a
+// time: the tile's cktab entries of this frame (coalesced 8-byte loads, four
+// per lane, those of the first 1024 issued when the tile starts) name the rows
+// whose span reaches into the tile.  A prefix sum over the row counts places
+// each entry's rows in a task table (entry, row), a window of kTaskCap tasks
+// at a time, and each lane takes table entries and walks one row from its
+// checkpoint (weights and the span inside the tile's column, at the slot of
+// triangle, column and row), with the triangle's dx, hz, flags and primitive
+// from its big record (two loads beside the checkpoint's).
+constexpr uint32_t kBigChunk = 1024;  // cktab entries per round, 4 per lane
+constexpr uint32_t kTaskCap = (uint32_t)(sizeof(TriLds) * kBatch / 2);  // tasks per window
+static_assert(kBigChunk * 8u <= (1u << 16), "tasks encode their entry and row in 16 bits");
 struct BigTaskLds {
-    uint32_t slot8[kBigChunk];        // checkpoint slot of the tile's row 0, + 8
-    uint16_t task[kBigChunk * 8u];    // entry << 3 | row of the tile, in entry order
+    uint16_t task[kTaskCap];  // entry << 3 | row of the tile
 };
 static_assert(sizeof(BigTaskLds) <= sizeof(TriLds) * kBatch, "big-triangle tasks fit the staging space");
 
-// Appends entry `e`'s rows (mask bits) to the task table from position `at`.
-EUCHIP_DEV void put_tasks(BigTaskLds &bt, uint32_t e, uint32_t mask, uint32_t at) {
-    for (; mask; mask &= mask - 1u) bt.task[at++] = (uint16_t)((e << 3) | (uint32_t)__builtin_ctz(mask));
+// Entry `e`'s rows (mask bits) are tasks at, at + 1, ...; those in the window
+// [w0, w0 + kTaskCap) go to the table.  Returns the next task number.
+EUCHIP_DEV uint32_t put_tasks(BigTaskLds &bt, uint32_t e, uint32_t mask, uint32_t at, uint32_t w0) {
+    for (; mask; mask &= mask - 1u, at++)
+        if (at - w0 < kTaskCap) bt.task[at - w0] = (uint16_t)((e << 3) | (uint32_t)__builtin_ctz(mask));
+    return at;
 }
 
 EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, uint32_t tile, uint32_t nbig,
-                          const TileGeom &g, uint32_t mode, unsigned long long e0, unsigned long long e1) {
+                          const TileGeom &g, uint32_t mode, const unsigned long long (&e_first)[4]) {
     BigTaskLds &bt = *reinterpret_cast<BigTaskLds *>(sh.tri);
     unsigned long long *trow = p.cktab + (size_t)tile * p.mstride;
+    const uint4 *cktile = p.ck + (size_t)(g.X0 >> g.sx) * p.v0.H + g.Y0;  // triangle t's at + t * ntx * H
+    const size_t ckstride = (size_t)p.v0.ntx * p.v0.H;
+    const uint32_t tag = ck_tag(p.frame);
     for (uint32_t s0 = 0; s0 < nbig; s0 += kBigChunk) {
-        if (s0) {
-            e0 = s0 + tid < nbig ? trow[s0 + tid] : 0ull;
-            e1 = s0 + 256u + tid < nbig ? trow[s0 + 256u + tid] : 0ull;
-        }
-        const uint32_t tag = ck_tag(p.frame);
-        const uint32_t m0 = (uint32_t)(e0 >> 40) == tag ? (uint32_t)(e0 >> 32) & 0xffu : 0u;
-        const uint32_t m1 = (uint32_t)(e1 >> 40) == tag ? (uint32_t)(e1 >> 32) & 0xffu : 0u;
-        // lane t holds entries t and 256 + t: one prefix sum over both counts,
-        // entry t's rows first, then entry 256 + t's after all of the first half
-        const uint32_t c0 = __builtin_popcount(m0), c1 = __builtin_popcount(m1);
-        const uint32_t i0 = block_inclusive_sum(sh, tid, c0);
-        const uint32_t t0 = sh.wave_rows[0] + sh.wave_rows[1] + sh.wave_rows[2] + sh.wave_rows[3];
-        __syncthreads();
-        const uint32_t i1 = block_inclusive_sum(sh, tid, c1) + t0;
-        const uint32_t total = t0 + sh.wave_rows[0] + sh.wave_rows[1] + sh.wave_rows[2] + sh.wave_rows[3];
-        bt.slot8[tid] = (uint32_t)e0, bt.slot8[256u + tid] = (uint32_t)e1;
-        put_tasks(bt, tid, m0, i0 - c0);
-        put_tasks(bt, 256u + tid, m1, i1 - c1);
-        __syncthreads();
-        // two rows per lane per turn, both rows' loads issued before either walk
-        for (uint32_t k0 = 0; k0 < total; k0 += 512u) {
-            uint4 c[2], x6[2], x7[2];
-            uint32_t r[2];
+        // lane t holds entries t, 256 + t, 512 + t and 768 + t
+        uint32_t m[4], mine = 0u;
 #pragma unroll
-            for (uint32_t h = 0; h < 2u; h++) {
-                const uint32_t k = k0 + h * 256u + tid;
-                r[h] = 8u;  // no row
-                if (k >= total) continue;
-                const uint32_t tk = bt.task[k], lo = tk >> 3;
-                r[h] = tk & 7u;
-                const uint4 *q = p.big_rec + (size_t)(s0 + lo) * kBigRecQuads;
-                c[h] = p.ck[(size_t)bt.slot8[lo] - 8u + r[h]];
-                x6[h] = q[6], x7[h] = q[7];
-            }
-#pragma unroll
-            for (uint32_t h = 0; h < 2u; h++) {
-                if (r[h] >= 8u) continue;
-                const uint32_t xs = g.X0 + (c[h].w & 0xffffu), xe = g.X0 + (c[h].w >> 16);
-                TriRec R;
-                R.dx[0] = __uint_as_float(x6[h].z), R.dx[1] = __uint_as_float(x6[h].w);
-                R.dx[2] = __uint_as_float(x7[h].x);
-                R.hz[0] = __uint_as_float(x7[h].y), R.hz[1] = __uint_as_float(x7[h].z);
-                R.hz[2] = __uint_as_float(x7[h].w);
-                R.flags = x6[h].y >> 30;
-                const uint32_t pi0 = r[h] * (g.TW + 1u) + (xs - g.X0);
-                const float w0 = __uint_as_float(c[h].x), w1 = __uint_as_float(c[h].y), w2 = __uint_as_float(c[h].z);
-                if (mode == 1u) walk_span<1u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-                else if (mode == 2u) walk_span<2u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-                else walk_span<0u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-            }
+        for (uint32_t i = 0; i < 4u; i++) {
+            const uint32_t at = s0 + i * 256u + tid;
+            const unsigned long long e = s0 ? (at < nbig ? trow[at] : 0ull) : e_first[i];
+            m[i] = (uint32_t)(e >> 40) == tag ? (uint32_t)(e >> 32) & 0xffu : 0u;
+            mine += __builtin_popcount(m[i]);
         }
-        __syncthreads();  // the next 512 reuse the task table and slots
+        const uint32_t first = block_inclusive_sum(sh, tid, mine) - mine;
+        const uint32_t total = sh.wave_rows[0] + sh.wave_rows[1] + sh.wave_rows[2] + sh.wave_rows[3];
+        for (uint32_t w0 = 0; w0 < total; w0 += kTaskCap) {
+            uint32_t at = first;
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; i++) at = put_tasks(bt, i * 256u + tid, m[i], at, w0);
+            __syncthreads();
+            const uint32_t n = total - w0 < kTaskCap ? total - w0 : kTaskCap;
+            // two rows per lane per turn, both rows' loads issued before either walk
+            for (uint32_t k0 = 0; k0 < n; k0 += 512u) {
+                uint4 c[2], x6[2], x7[2];
+                uint32_t r[2];
+#pragma unroll
+                for (uint32_t h = 0; h < 2u; h++) {
+                    const uint32_t k = k0 + h * 256u + tid;
+                    r[h] = 8u;  // no row
+                    if (k >= n) continue;
+                    const uint32_t tk = bt.task[k], lo = tk >> 3;
+                    r[h] = tk & 7u;
+                    const uint4 *q = p.big_rec + (size_t)(s0 + lo) * kBigRecQuads;
+                    c[h] = cktile[(s0 + lo) * ckstride + r[h]];
+                    x6[h] = q[0], x7[h] = q[1];
+                }
+#pragma unroll
+                for (uint32_t h = 0; h < 2u; h++) {
+                    if (r[h] >= 8u) continue;
+                    const uint32_t xs = g.X0 + (c[h].w & 0xffffu), xe = g.X0 + (c[h].w >> 16);
+                    TriRec R;
+                    R.dx[0] = __uint_as_float(x6[h].z), R.dx[1] = __uint_as_float(x6[h].w);
+                    R.dx[2] = __uint_as_float(x7[h].x);
+                    R.hz[0] = __uint_as_float(x7[h].y), R.hz[1] = __uint_as_float(x7[h].z);
+                    R.hz[2] = __uint_as_float(x7[h].w);
+                    R.flags = x6[h].y >> 30;
+                    const uint32_t pi0 = r[h] * (g.TW + 1u) + (xs - g.X0);
+                    const float w0 = __uint_as_float(c[h].x), w1 = __uint_as_float(c[h].y);
+                    const float w2 = __uint_as_float(c[h].z);
+                    if (mode == 1u) walk_span<1u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+                    else if (mode == 2u) walk_span<2u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+                    else walk_span<0u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+                }
+            }
+            __syncthreads();  // the next window reuses the task table
+        }
+        __syncthreads();  // the next round's prefix sum reuses the wave totals
     }
 }
 
@@ -1140,7 +1137,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     const uint32_t flags = p.ctr->flags[par][0];
     const uint32_t pool_used0 = p.ctr->pool[par][0];
     uint32_t nh = __builtin_amdgcn_readfirstlane(p.ctr->heavy[par][0]);
-    uint32_t nbig = BIG ? __builtin_amdgcn_readfirstlane(p.ctr->big[par][0]) : 0u;
+    // big triangles: with checkpoints every triangle has a (tagged) entry per tile
+    uint32_t nbig = !BIG ? 0u : p.ck_on ? p.v0.n_tris : __builtin_amdgcn_readfirstlane(p.ctr->big[par][0]);
     uint32_t nbusy = __builtin_amdgcn_readfirstlane(p.ctr->busy[par][xcd][0]);
     const uint32_t first = w < lcap ? __builtin_amdgcn_readfirstlane(list[w]) : 0u;
     const TileBin first_bin = p.tile_count[first < ntiles ? first : 0u];
@@ -1150,8 +1148,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         rp->max_count = p.ctr->max_count[par][0];
         rp->flags = flags;
         rp->pool_used = pool_used0;
-        const unsigned long long ck_used = p.ctr->ck[par][0];
-        rp->ck_used = ck_used < 0xFFFFFFFFull ? (uint32_t)ck_used : 0xFFFFFFFFu;
         if (flags & FLAG_BIN_OVERFLOW) rp->dropped = rp->dropped + 1u;
         __threadfence_system();
         rp->frame = p.frame;
@@ -1163,7 +1159,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         p.ctr->max_count[npar][0] = 0u;
         p.ctr->heavy[npar][0] = 0u;
         p.ctr->big[npar][0] = 0u;
-        p.ctr->ck[npar][0] = 0ull;
     }
     if (blockIdx.x < 8u && tid == 0u) {
         p.ctr->next[npar][xcd][0] = 0u;
@@ -1180,7 +1175,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     const uint32_t sx = p.tile_sx, sy = 8u - sx;
     uint32_t *next = &p.ctr->next[par][xcd][0];
     // big triangles from their row checkpoints (else walked from the row starts)
-    const bool ckp = BIG && p.ck_cap && !(flags & FLAG_CK_MISS);
+    const bool ckp = BIG && p.ck_on;
 
     // One tile with `cnt` bin entries and overflow list `head`: winner keys in
     // LDS, then the resolve.
@@ -1217,12 +1212,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         }
         const uint32_t cap = v.tile_cap;
         const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
-        // the tile's first 512 checkpoint entries, loaded now (used after the bin)
-        unsigned long long e0 = 0ull, e1 = 0ull;
+        // the tile's first 1024 checkpoint entries, loaded now (used after the bin)
+        unsigned long long e_first[4] = {0ull, 0ull, 0ull, 0ull};
         if (BIG && ckp) {
             const unsigned long long *trow = p.cktab + (size_t)tile * p.mstride;
-            if (tid < nbig) e0 = trow[tid];
-            if (256u + tid < nbig) e1 = trow[256u + tid];
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; i++)
+                if (i * 256u + tid < nbig) e_first[i] = trow[i * 256u + tid];
         }
         __syncthreads();
         // Runs of entries, kBatch at a time: this tile's bin and the big
@@ -1251,20 +1247,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             if (b >= run) break;
             const uint32_t i = b + tid;
             bool valid = false;
-            uint32_t prim = 0, big = ~0u;
+            uint32_t prim = 0;
             if (tid >= kBatch || i >= run) {
             } else if (base) {
                 valid = true, prim = p.pool[base + i];
             } else if (i < nb) {
                 valid = true, prim = bin[i];
             } else if constexpr (BIG) {
-                valid = true;
-                if (p.ck_cap) big = i - nb;
-                else prim = p.big[i - nb];
+                valid = true, prim = p.big[i - nb];
             }
-            raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, prim, big, g, mode);
+            raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, prim, g, mode);
         }
-        if (BIG && ckp && nbig) big_tasks(p, sh, tid, tile, nbig, g, mode, e0, e1);
+        if (BIG && ckp && nbig) big_tasks(p, sh, tid, tile, nbig, g, mode, e_first);
         uint32_t ntile = 0u;
         TileBin ntb{0u, 0u};
         if (BATCH && claim && tid == 0u && claimed < nbusy) {
@@ -1327,7 +1321,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         __syncthreads();
     };
 
-    if constexpr (BIG) nbig = nbig < p.big_cap ? nbig : p.big_cap;
+    if (BIG && !p.ck_on) nbig = nbig < p.big_cap ? nbig : p.big_cap;
     // With big triangles every tile may hold fragments: the loop below then takes
     // every tile of this XCD's contiguous range instead of its busy list.
     if (BIG && nbig) {
@@ -1417,7 +1411,17 @@ static void launch_setup_t(const FrameParams &p, hipStream_t s, bool batched, ui
         hipLaunchKernelGGL((k_setup<SH, true>), dim3(blocks), dim3(256), 0, s, p);
     } else {
         const size_t n = std::max<size_t>(p.v0.n_tris, (size_t)p.v0.n_instances * p.v0.n_vertices);
-        hipLaunchKernelGGL((k_setup<SH, false>), dim3(blocks_of(n, 256)), dim3(256), 0, s, p);
+        FrameParams q = p;
+        q.setup_blocks = blocks_of(n, 256);
+        uint32_t span_blocks = 0;
+        size_t lds = 0;
+        if (p.ck_on && p.v0.H) {  // span blocks: 4 items of kSpanRows rows per triangle and block
+            const uint32_t sy = 8u - p.tile_sx;
+            q.span_parts = (p.v0.H + (1u << sy) + kSpanRows - 1u) / kSpanRows;
+            span_blocks = p.v0.n_tris * ((q.span_parts + 3u) / 4u);
+            lds = 4u * 4u * (((kSpanRows >> sy) * p.v0.ntx + 3u) / 4u);
+        }
+        hipLaunchKernelGGL((k_setup<SH, false>), dim3(q.setup_blocks + span_blocks), dim3(256), lds, s, q);
     }
 }
 
@@ -1489,41 +1493,41 @@ static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, u
 }
 
 // ---------------------------------------------------------------------------
-// k_span: row checkpoints and cktab entries of the big triangles (layout in
-// internal.hpp).  An item is 64 rows of one big triangle, counted from its
-// first row rounded down to a tile row (so every tile row belongs to one
-// item); workgroup b takes triangle b % nslots and its wave w item
-// 4 * (b / nslots) + w.  A lane takes a row: the row span (triangles.rs:219-254,
-// with the row's band), a bit in the wave's LDS mask table for every tile
-// column the span touches, then the walk from x0 (triangles.rs:257-301, the
-// same w += dx additions as the raster's walk) storing the weights where the
-// span enters each column.  The wave then writes the entries of its tiles.
+// Span blocks of a checkpoint frame's setup launch: block b walks rows of
+// triangle t = b % n_tris, its wave w the item 4 * (b / n_tris) + w (kSpanRows
+// rows from the triangle's first tile row).  The wave sets its triangle up
+// itself (tri_setup, as the setup blocks do) and leaves it to them unless its
+// box covers more than kBigTiles tiles.  A lane takes a row: the row span
+// (triangles.rs:219-254, with the row's band), a bit in the wave's LDS mask
+// table for every tile column the span touches, then the walk from x0
+// (triangles.rs:257-301, the same w += dx additions as the raster's walk)
+// storing the weights where the span enters each column.  The wave then writes
+// the entries of its tiles.  Slots and entries sit at fixed places (layout in
+// internal.hpp), so no block waits for another.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_span(FrameParams p, uint32_t nslots, uint32_t parts) {
+EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b) {
     extern __shared__ uint32_t span_lds[];  // per wave: (kSpanRows >> sy) x ntx mask bytes
-    const uint32_t par = p.frame & 1u;
-    const uint32_t flags = p.ctr->flags[par][0];
-    if (flags & (FLAG_BIN_OVERFLOW | FLAG_CK_MISS)) return;
-    uint32_t nbig = __builtin_amdgcn_readfirstlane(p.ctr->big[par][0]);
-    nbig = nbig < p.big_cap ? nbig : p.big_cap;
-    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    const uint32_t s = blockIdx.x % nslots, part = (blockIdx.x / nslots) * 4u + wv;
-    if (s >= nbig || part >= parts) return;
     const View &v = p.v0;
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t t = b % v.n_tris, part = (b / v.n_tris) * 4u + wv;
+    if (part >= p.span_parts) return;
+    TriRec R;
+    if (!tri_setup(p, v, t, R)) return;
     const uint32_t sx = p.tile_sx, sy = 8u - sx, TH = 1u << sy;
+    const uint32_t col0 = R.xmin >> sx;
+    const uint32_t cols = ((R.xmax - 1u) >> sx) - col0 + 1u;
+    if (cols * (((R.ymax - 1u) >> sy) - (R.ymin >> sy) + 1u) <= kBigTiles) return;  // binned
+    if (part == 0u && lane == 0u) {  // what the raster's row tasks read besides the slot
+        uint4 *dst = p.big_rec + (size_t)t * kBigRecQuads;
+        dst[0] = make_uint4(t, R.flags << 30, __float_as_uint(R.dx[0]), __float_as_uint(R.dx[1]));
+        dst[1] = make_uint4(__float_as_uint(R.dx[2]), __float_as_uint(R.hz[0]), __float_as_uint(R.hz[1]),
+                            __float_as_uint(R.hz[2]));
+    }
+    const uint32_t ybase = ((R.ymin >> sy) << sy) + part * kSpanRows;
+    if (ybase >= R.ymax) return;
     const uint32_t groups = kSpanRows >> sy;
     uint32_t *mw = span_lds + wv * ((groups * v.ntx + 3u) >> 2);
     const uint8_t *mb = reinterpret_cast<const uint8_t *>(mw);
-    const uint4 *bq = p.big_rec + (size_t)s * kBigRecQuads;
-    TriRec R;
-    load_rec0_at(bq, R);
-    load_rec1_at(bq + kRec0Quads, R);
-    const uint32_t ck = __builtin_amdgcn_readfirstlane(bq[6].y) & kCkNone;
-    const uint32_t ybase = ((R.ymin >> sy) << sy) + part * kSpanRows;
-    if (ck == kCkNone || ybase >= R.ymax) return;
-    const uint32_t col0 = R.xmin >> sx;
-    const uint32_t cols = ((R.xmax - 1u) >> sx) - col0 + 1u;
-    const uint32_t rows = R.ymax - R.ymin;
     for (uint32_t i = lane; i < ((groups * cols + 3u) >> 2); i += 64u) mw[i] = 0u;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1544,12 +1548,12 @@ __global__ __launch_bounds__(256) void k_span(FrameParams p, uint32_t nslots, ui
         float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
         float w1 = (R.o[1] + R.dy[1] * Y) + R.dx[1] * X0;
         float w2 = (R.o[2] + R.dy[2] * Y) + R.dx[2] * X0;
-        uint4 *row = p.ck + (size_t)ck + (y - R.ymin);
+        uint4 *row = p.ck + (size_t)t * v.ntx * v.H + y;  // column c's slot at row + c * H
         for (uint32_t x = x0;;) {
             const uint32_t c = x >> sx, xc = c << sx;
             const uint32_t xe = xc + (1u << sx) < x1 ? xc + (1u << sx) : x1;
-            row[(size_t)rows * (c - col0)] = make_uint4(__float_as_uint(w0), __float_as_uint(w1),
-                                                        __float_as_uint(w2), (x - xc) | ((xe - xc) << 16));
+            row[(size_t)c * v.H] = make_uint4(__float_as_uint(w0), __float_as_uint(w1), __float_as_uint(w2),
+                                              (x - xc) | ((xe - xc) << 16));
             if (xe >= x1) break;
             for (; x + 8u <= xe; x += 8u) {
 #pragma unroll
@@ -1570,24 +1574,13 @@ __global__ __launch_bounds__(256) void k_span(FrameParams p, uint32_t nslots, ui
     __builtin_amdgcn_wave_barrier();
     // the entries of this item's tiles (every tile row of it is this item's)
     const uint32_t ty0 = ybase >> sy;
+    const unsigned long long tag = (unsigned long long)ck_tag(p.frame) << 40;
     for (uint32_t i = lane; i < groups * cols; i += 64u) {
-        const uint32_t b = mb[i];
-        if (b == 0u) continue;
+        const uint32_t m = mb[i];
+        if (m == 0u) continue;
         const uint32_t gi = i / cols, c = i - gi * cols;
-        const uint32_t slot8 = ck + c * rows + ((ty0 + gi) << sy) + 8u - R.ymin;  // tile row 0's slot + 8
-        p.cktab[((size_t)(ty0 + gi) * v.ntx + col0 + c) * p.mstride + s] =
-            ((unsigned long long)ck_tag(p.frame) << 40) | ((unsigned long long)b << 32) | slot8;
+        p.cktab[((size_t)(ty0 + gi) * v.ntx + col0 + c) * p.mstride + t] = tag | ((unsigned long long)m << 32);
     }
-}
-
-hipError_t launch_span(const FrameParams &p, hipStream_t s) {
-    if (!p.big_cap || !p.ck_cap || p.v0.H == 0u) return hipSuccess;
-    const uint32_t sy = 8u - p.tile_sx;
-    const uint32_t parts = (p.v0.H + (1u << sy) + kSpanRows - 1u) / kSpanRows;
-    const uint32_t nslots = std::max<uint32_t>(1u, std::min<uint32_t>(p.big_cap, p.v0.n_tris));
-    const size_t lds = 4u * 4u * (((kSpanRows >> sy) * p.v0.ntx + 3u) / 4u);
-    hipLaunchKernelGGL(k_span, dim3(nslots * ((parts + 3u) / 4u)), dim3(256), lds, s, p, nslots, parts);
-    return hipGetLastError();
 }
 
 // Workgroups of the tile loop: a multiple of the 8 XCDs, no more than the

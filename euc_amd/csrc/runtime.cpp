@@ -68,10 +68,9 @@ struct euchip_ctx {
     DevArray<uint32_t> heavy;   // heavy-tile list
     DevArray<uint32_t> busy;    // busy-tile lists (one per XCD tile range)
     DevArray<uint32_t> big;     // big-triangle list
-    DevArray<uint4> big_rec;    // per big triangle: setup record and first checkpoint slot
+    DevArray<uint4> big_rec;    // per triangle of a checkpoint frame: what its row tasks read
     DevArray<uint4> ck;         // row checkpoints of big triangles
-    DevArray<unsigned long long> cktab;  // their tagged row masks and slots per (tile, big triangle)
-    size_t ck_want = 0;         // checkpoint slots the next big-triangle frames should have (from the reports)
+    DevArray<unsigned long long> cktab;  // their tagged row masks per (tile, triangle)
     DevArray<uint32_t> lrec;    // line records
     DevArray<uint64_t> lkey;    // per-pixel winner keys of a lines frame
     std::vector<Deferred> deferred;
@@ -494,8 +493,6 @@ void absorb_report(euchip_ctx *ctx) {
         ctx->tile_cap = c;
     }
     ctx->pool_want = std::max<size_t>(ctx->pool_want, (size_t)used + used / 4);
-    const size_t ck = r->ck_used;
-    ctx->ck_want = std::min<size_t>(std::max<size_t>(ctx->ck_want, ck + ck / 4), kCkMaxSlots);
 }
 
 // render() of one request of lines (lines.rs:15-121) on the context's stream.
@@ -579,18 +576,19 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     const size_t big_cap = std::min<size_t>(std::max<size_t>(P.n_tris, 1), kBigCap);
     if ((s = ensure(ctx, ctx->big, big_cap, "big triangles", ctx->stream))) return s;
     // Row checkpoints (big-triangle frames): optional, so a pool the budget
-    // cannot hold leaves the big triangles on the walk from the row start.
-    // Tiles at most 8 rows high (an 8-bit row mask per tile and big triangle),
-    // at most 64 MB of tile entries, and the checkpoint pass's LDS mask tables
-    // (4 waves x 64 rows x tile columns bytes) within 64 KB.
+    // cannot hold leaves the big triangles on the big list, walked from the row
+    // starts.  Tiles at most 8 rows high (an 8-bit row mask per tile and
+    // triangle), at most 64 MB of tile entries and 1 GiB of checkpoints (one
+    // slot per triangle, tile column and row), and the span blocks' LDS mask
+    // tables (4 waves x 64 rows x tile columns bytes) within 64 KB.
     bool ck_on = false;
-    const size_t mstride = (big_cap + 3) & ~(size_t)3;
-    if (big_mode && p.tile_sx >= 5 && ntiles * mstride <= ((size_t)8 << 20) &&
+    const size_t mstride = ((size_t)P.n_tris + 3) & ~(size_t)3;
+    const size_t ck_slots = (size_t)P.n_tris * p.v0.ntx * p.v0.H;
+    if (big_mode && p.tile_sx >= 5 && ntiles * mstride <= ((size_t)8 << 20) && ck_slots <= kCkMaxSlots &&
         4u * (kSpanRows >> (8u - p.tile_sx)) * (size_t)p.v0.ntx <= ((size_t)64 << 10)) {
         const size_t mask_had = ctx->cktab.cap;
-        ck_on = ensure(ctx, ctx->big_rec, big_cap * kBigRecQuads, "big triangles", ctx->stream) == EUCHIP_OK &&
-                ensure(ctx, ctx->ck, std::max<size_t>(ctx->ck_want, kCkInitSlots), "row checkpoints",
-                       ctx->stream) == EUCHIP_OK &&
+        ck_on = ensure(ctx, ctx->big_rec, (size_t)P.n_tris * kBigRecQuads, "big triangles", ctx->stream) == EUCHIP_OK &&
+                ensure(ctx, ctx->ck, ck_slots, "row checkpoints", ctx->stream) == EUCHIP_OK &&
                 ensure(ctx, ctx->cktab, ntiles * mstride, "row masks", ctx->stream) == EUCHIP_OK;
         if (ck_on && ctx->cktab.cap != mask_had)
             HIP_TRY(ctx, hipMemsetAsync(ctx->cktab.ptr, 0, ctx->cktab.cap * 8u, ctx->stream));
@@ -615,10 +613,9 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.big = ctx->big.ptr;
         p.big_cap = big_mode ? (uint32_t)std::min<size_t>(ctx->big.cap, kBigCap) : 0u;
         if (ck_on) {
-            p.big_cap = (uint32_t)std::min<size_t>(p.big_cap, ctx->big_rec.cap / kBigRecQuads);
             p.big_rec = ctx->big_rec.ptr;
             p.ck = ctx->ck.ptr;
-            p.ck_cap = (uint32_t)std::min<size_t>(ctx->ck.cap, kCkMaxSlots);
+            p.ck_on = 1u;
             p.cktab = ctx->cktab.ptr;
             p.mstride = (uint32_t)mstride;
         }
@@ -630,10 +627,9 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.heavy = ctx->heavy.ptr;
         p.heavy_cap = (uint32_t)std::min<size_t>(ctx->heavy.cap, 0xFFFFFFF0ull);
         p.frame = ctx->frame++;
-        if (p.ck_cap && p.frame % 0xffffffu == 0u)  // the tags start over: no old entry may match
+        if (p.ck_on && p.frame % 0xffffffu == 0u)  // the tags start over: no old entry may match
             HIP_TRY(ctx, hipMemsetAsync(ctx->cktab.ptr, 0, ctx->cktab.cap * 8u, ctx->stream));
         if ((s = timed(ctx, "setup", [&] { return launch_setup(p, ctx->stream, false, 0); }))) return s;
-        if (p.ck_cap && (s = timed(ctx, "span", [&] { return launch_span(p, ctx->stream); }))) return s;
         if ((s = timed(ctx, "raster", [&] { return launch_raster(p, ctx->stream, false, ctx->max_groups); }))) return s;
         ctx->counts_used[npar] = 0;
         ctx->counts_used[par] = (uint32_t)ntiles;

@@ -1,12 +1,12 @@
 """Big-triangle mode on the GPU against the oracle, bit-exact: meshes with no more
 triangles than the target has tiles, whose triangles cover many tiles each.
 
-Their rows are walked once by the checkpoint pass and resumed in every tile
-(triangles.rs:257-301 additions, unchanged); these cases force the other paths
-that mode has:
-  * a first frame whose big triangles need more checkpoint slots than a fresh
-    context holds (they then walk every row from its start), and the second frame
-    of the same context, which gets the pool that frame asked for;
+Their rows are walked once by the span blocks of the setup launch and resumed
+in every tile (triangles.rs:257-301 additions, unchanged); these cases force the
+other paths that mode has:
+  * large checkpoint frames (2048^2 with triangles spanning the target), and
+    frames whose checkpoint tables would pass their limits, whose big triangles
+    go on the big list and are walked from the row starts;
   * every tile shape the context takes for big-triangle frames: 64x4, 128x2 and
     256x1 tiles use the checkpoints with 4-, 2- and 1-row masks, 8x32 and 16x16
     tiles (taller than 8 rows) walk from the row starts;
@@ -24,7 +24,8 @@ from tests.parity import assert_bit_exact, run_pair
 
 pytestmark = pytest.mark.gpu
 
-CK_INIT_SLOTS = 1 << 21  # checkpoint slots of a fresh context
+CK_MAX_SLOTS = 1 << 26   # checkpoint slots a frame may take (one per triangle, tile column and row)
+CKTAB_MAX = 8 << 20      # tile entries a frame may take (one per tile and triangle)
 
 
 def corner_triangles(seed, n):
@@ -43,18 +44,25 @@ def corner_triangles(seed, n):
     return S.Mesh(pos=pos, count=3 * n, rgba8=rgba8)
 
 
-def checkpoint_slots(mesh, w, h, sx=5):
-    """Slots the checkpoint pass asks for: rows x tile columns of every triangle's
-    clamped box (as triangle setup computes it)."""
-    p = mesh.pos.reshape(-1, 3, 4).astype(np.float64)
-    xs = (p[:, :, 0] / p[:, :, 3] * 0.5 + 0.5) * w
-    ys = (p[:, :, 1] / p[:, :, 3] * -0.5 + 0.5) * h
-    xmin = np.clip(np.floor(xs.min(1)), 0, w).astype(np.int64)
-    xmax = np.clip(np.floor(xs.max(1) + 1), 0, w).astype(np.int64)
-    ymin = np.clip(np.floor(ys.min(1)), 0, h).astype(np.int64)
-    ymax = np.clip(np.floor(ys.max(1) + 1), 0, h).astype(np.int64)
-    cols = ((xmax - 1) >> sx) - (xmin >> sx) + 1
-    return int(((ymax - ymin) * cols).sum())
+def mid_triangles(seed, n, half=0.2):
+    """n triangles whose vertices lie within `half` (NDC) of a random centre
+    (boxes of about 100 tiles at 1024^2), z in [0, 1], w = 1, one flat colour each."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-0.8, 0.8, (n, 1, 2))
+    pos = np.empty((n, 3, 4), np.float32)
+    pos[:, :, :2] = c + rng.uniform(-half, half, (n, 3, 2))
+    pos[:, :, 2] = rng.uniform(0.0, 1.0, (n, 3))
+    pos[:, :, 3] = 1.0
+    rgba8 = np.repeat(rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32) | np.uint32(0xFF000000), 3)
+    return S.Mesh(pos=pos.reshape(-1, 4), count=3 * n, rgba8=rgba8)
+
+
+def checkpoint_on(n_tris, w, h, sx=5):
+    """Whether a big-triangle frame keeps row checkpoints: its slots (triangles x
+    tile columns x rows) and tile entries (tiles x triangles, rounded to 4)
+    within their limits."""
+    ntx, nty = -(-w // (1 << sx)), -(-h // (256 >> sx))
+    return n_tris * ntx * h <= CK_MAX_SLOTS and ntx * nty * ((n_tris + 3) & ~3) <= CKTAB_MAX
 
 
 def flat(dm=DepthMode.LESS_WRITE, bands=BandPolicy.PAR, aa=AaMode.NONE):
@@ -62,17 +70,34 @@ def flat(dm=DepthMode.LESS_WRITE, bands=BandPolicy.PAR, aa=AaMode.NONE):
                              bands)
 
 
-def test_checkpoint_pool_too_small_then_grown():
+def test_large_checkpoint_frame():
     from euc_amd.renderer import Context
     w = h = 2048
     mesh = corner_triangles(7, 28)
-    assert checkpoint_slots(mesh, w, h) > CK_INIT_SLOTS + CK_INIT_SLOTS // 4  # the first frame cannot hold them
+    assert checkpoint_on(28, w, h)
     ctx = Context(0)
     try:
-        for frame in range(2):  # the pool grows from the first frame's report
+        for frame in range(2):
             r = run_pair(ctx, flat(), S.uniforms(), mesh, w, h, color_init=0xFF000000)
             assert r["frags"] > 10 ** 6
             assert_bit_exact(r, f"2048^2 corner triangles, frame {frame}")
+    finally:
+        ctx.close()
+
+
+def test_big_list_without_checkpoints():
+    # 2,100 triangles of about 100 tiles each on 1024^2 (4,096 tiles): big-triangle
+    # mode, but 4,096 x 2,100 tile entries pass their limit, so the big
+    # triangles go on the big list and every tile walks them from the row starts
+    from euc_amd.renderer import Context
+    w = h = 1024
+    mesh = mid_triangles(17, 2100)
+    assert not checkpoint_on(2100, w, h)
+    ctx = Context(0)
+    try:
+        r = run_pair(ctx, flat(), S.uniforms(), mesh, w, h, color_init=0xFF000000)
+        assert r["frags"] > 10 ** 6
+        assert_bit_exact(r, "1024^2, 2100 mid-size triangles from the big list")
     finally:
         ctx.close()
 
