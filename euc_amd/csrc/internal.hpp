@@ -115,16 +115,11 @@ constexpr uint32_t kBigCap = 4096;
 // + y of the pool (one 16-byte slot per triangle, column and row of the
 // target, so the 8 rows of one tile are consecutive): (w0, w1, w2) at xs, then
 // (xs - X0) | (xe - X0) << 16, the row's span inside the column that starts at
-// X0.  Per (tile, triangle) one 8-byte entry of cktab (tile * mstride + t)
-// holds, when some row of the tile has such a span, the frame's tag (bits
-// 40-63, ck_tag below) and the mask of those rows (bits 32-39); an entry with
-// another tag is stale.  The raster pass reads its tile's entries and walks the
-// tagged rows from their checkpoints.  Tiles are then at most 8 rows high.
+// X0.  Each tile some row of which has such a span gets the 4-byte entry
+// t << 8 | (mask of those rows) in its list (cktab, tile * mstride + k, k from
+// the tile's count in ck_count).  The raster pass reads its tile's list and
+// walks those rows from their checkpoints.  Tiles are then at most 8 rows high.
 constexpr uint32_t kSpanRows = 64;             // rows per item of the checkpoint walk (one wave)
-// The tag of frame f in cktab, 1..0xffffff; the host clears the table before
-// each frame whose f % 0xffffff is 0, so an old entry never carries the tag of
-// a later frame.
-__host__ __device__ inline uint32_t ck_tag(uint32_t frame) { return frame % 0xffffffu + 1u; }
 // Per triangle of a checkpoint frame (big_rec): (primitive, flags << 30, dx0,
 // dx1), (dx2, hz0, hz1, hz2), what the raster's row tasks read besides the slot.
 constexpr uint32_t kBigRecQuads = 2;
@@ -206,15 +201,20 @@ struct FrameParams {
     uint32_t *big;         // big triangles (primitive ids), big_cap slots; big_cap 0: off (batches)
     uint32_t big_cap;
     // Row checkpoints (ck_on): per triangle its big record (kBigRecQuads
-    // quads), the pool of n_tris x ntx x H slots and the tagged row masks
-    // (ntiles x mstride entries, mstride >= n_tris); the setup launch then has
-    // setup_blocks setup blocks, then the span blocks (span_parts items of
-    // kSpanRows rows per triangle, 4 per block).
+    // quads), the pool of n_tris x ntx x H slots and the tiles' lists
+    // (ntiles x mstride entries, mstride >= n_tris) with their counts (two
+    // halves by frame parity like tile_count: ck_count this frame's,
+    // ck_count_next the other, whose first ck_zero_n counts a big-triangle
+    // frame's raster zeroes);
+    // the setup launch then has setup_blocks setup blocks, then the span
+    // blocks (span_parts items of kSpanRows rows per triangle, 4 per block).
     uint4 *big_rec;
     uint4 *ck;
     uint32_t ck_on;
-    unsigned long long *cktab;
+    uint32_t *cktab;
     uint32_t mstride;
+    uint32_t *ck_count, *ck_count_next;
+    uint32_t ck_zero_n;
     uint32_t setup_blocks, span_parts;
     TileBin *count_next;
     uint32_t zero_n;

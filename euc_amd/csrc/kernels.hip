@@ -480,13 +480,15 @@ union SetupShared {
     } bin;
 };
 
-template <uint32_t SH, bool BATCH>
+// SPAN: the launch of a checkpoint frame, whose blocks past setup_blocks are
+// span blocks (compiled apart so other frames' setup carries none of their code).
+template <uint32_t SH, bool BATCH, bool SPAN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_setup(FrameParams p) {
     __shared__ SetupShared sm;
     __shared__ uint32_t npl;
     __shared__ uint32_t lcnt[9], lbase[9];  // new busy tiles per list (8) and new heavy tiles (8)
-    if constexpr (!BATCH) {
-        if (blockIdx.x >= p.setup_blocks) {  // a span block of a checkpoint frame
+    if constexpr (SPAN) {
+        if (blockIdx.x >= p.setup_blocks) {
             span_item(p, blockIdx.x - p.setup_blocks);
             return;
         }
@@ -512,10 +514,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
             ty0 = R.ymin >> sy;
             nx = ((R.xmax - 1u) >> sx) - tx0 + 1u;
             ny = ((R.ymax - 1u) >> sy) - ty0 + 1u;
-            if (nx * ny > kBigTiles) {  // not binned
+            if ((p.ck_on || p.big_cap) && nx * ny > kBigTiles) {  // not binned
                 if (p.ck_on) {
                     nx = ny = 0u;  // the span blocks walk it
-                } else if (p.big_cap) {  // on the big-triangle list (while it has room)
+                } else {  // on the big-triangle list (while it has room)
                     const uint32_t k = atomicAdd(&p.ctr->big[p.frame & 1u][0], 1u);
                     if (k < p.big_cap) {
                         p.big[k] = t;
@@ -1021,92 +1023,69 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
     __syncthreads();  // the next batch reuses sh.tri and sh.rows_incl
 }
 
-// The big triangles of `tile` from their row checkpoints, 1024 triangles at a
-// time: the tile's cktab entries of this frame (coalesced 8-byte loads, four
-// per lane, those of the first 1024 issued when the tile starts) name the rows
-// whose span reaches into the tile.  A prefix sum over the row counts places
-// each entry's rows in a task table (entry, row), a window of kTaskCap tasks
-// at a time, and each lane takes table entries and walks one row from its
+// The big triangles of `tile` from their row checkpoints: the tile's list of
+// (triangle, rows whose span reaches into the tile), 256 entries at a time
+// (one per lane, coalesced; the first 256 issued when the tile starts).  A
+// prefix sum over the row counts places each entry's rows in a task table
+// (entry, row), and each lane takes table entries and walks one row from its
 // checkpoint (weights and the span inside the tile's column, at the slot of
 // triangle, column and row), with the triangle's dx, hz, flags and primitive
 // from its big record (two loads beside the checkpoint's).
-constexpr uint32_t kBigChunk = 1024;  // cktab entries per round, 4 per lane
-constexpr uint32_t kTaskCap = (uint32_t)(sizeof(TriLds) * kBatch / 2);  // tasks per window
-static_assert(kBigChunk * 8u <= (1u << 16), "tasks encode their entry and row in 16 bits");
 struct BigTaskLds {
-    uint16_t task[kTaskCap];  // entry << 3 | row of the tile
+    uint32_t tri[256];        // triangle of each entry of the round
+    uint16_t task[256 * 8];   // entry << 3 | row of the tile
 };
 static_assert(sizeof(BigTaskLds) <= sizeof(TriLds) * kBatch, "big-triangle tasks fit the staging space");
 
-// Entry `e`'s rows (mask bits) are tasks at, at + 1, ...; those in the window
-// [w0, w0 + kTaskCap) go to the table.  Returns the next task number.
-EUCHIP_DEV uint32_t put_tasks(BigTaskLds &bt, uint32_t e, uint32_t mask, uint32_t at, uint32_t w0) {
-    for (; mask; mask &= mask - 1u, at++)
-        if (at - w0 < kTaskCap) bt.task[at - w0] = (uint16_t)((e << 3) | (uint32_t)__builtin_ctz(mask));
-    return at;
-}
-
 EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, uint32_t tile, uint32_t nbig,
-                          const TileGeom &g, uint32_t mode, const unsigned long long (&e_first)[4]) {
+                          const TileGeom &g, uint32_t mode, uint32_t e_first) {
     BigTaskLds &bt = *reinterpret_cast<BigTaskLds *>(sh.tri);
-    unsigned long long *trow = p.cktab + (size_t)tile * p.mstride;
+    const uint32_t *list = p.cktab + (size_t)tile * p.mstride;
     const uint4 *cktile = p.ck + (size_t)(g.X0 >> g.sx) * p.v0.H + g.Y0;  // triangle t's at + t * ntx * H
     const size_t ckstride = (size_t)p.v0.ntx * p.v0.H;
-    const uint32_t tag = ck_tag(p.frame);
-    for (uint32_t s0 = 0; s0 < nbig; s0 += kBigChunk) {
-        // lane t holds entries t, 256 + t, 512 + t and 768 + t
-        uint32_t m[4], mine = 0u;
-#pragma unroll
-        for (uint32_t i = 0; i < 4u; i++) {
-            const uint32_t at = s0 + i * 256u + tid;
-            const unsigned long long e = s0 ? (at < nbig ? trow[at] : 0ull) : e_first[i];
-            m[i] = (uint32_t)(e >> 40) == tag ? (uint32_t)(e >> 32) & 0xffu : 0u;
-            mine += __builtin_popcount(m[i]);
-        }
-        const uint32_t first = block_inclusive_sum(sh, tid, mine) - mine;
+    for (uint32_t s0 = 0; s0 < nbig; s0 += 256u) {
+        const uint32_t e = s0 ? (s0 + tid < nbig ? list[s0 + tid] : 0u) : e_first;
+        const uint32_t m = s0 + tid < nbig ? e & 0xffu : 0u;
+        const uint32_t mine = __builtin_popcount(m);
+        uint32_t at = block_inclusive_sum(sh, tid, mine) - mine;
         const uint32_t total = sh.wave_rows[0] + sh.wave_rows[1] + sh.wave_rows[2] + sh.wave_rows[3];
-        for (uint32_t w0 = 0; w0 < total; w0 += kTaskCap) {
-            uint32_t at = first;
+        bt.tri[tid] = e >> 8;
+        for (uint32_t mm = m; mm; mm &= mm - 1u) bt.task[at++] = (uint16_t)((tid << 3) | (uint32_t)__builtin_ctz(mm));
+        __syncthreads();
+        // two rows per lane per turn, both rows' loads issued before either walk
+        for (uint32_t k0 = 0; k0 < total; k0 += 512u) {
+            uint4 c[2], x6[2], x7[2];
+            uint32_t r[2];
 #pragma unroll
-            for (uint32_t i = 0; i < 4u; i++) at = put_tasks(bt, i * 256u + tid, m[i], at, w0);
-            __syncthreads();
-            const uint32_t n = total - w0 < kTaskCap ? total - w0 : kTaskCap;
-            // two rows per lane per turn, both rows' loads issued before either walk
-            for (uint32_t k0 = 0; k0 < n; k0 += 512u) {
-                uint4 c[2], x6[2], x7[2];
-                uint32_t r[2];
-#pragma unroll
-                for (uint32_t h = 0; h < 2u; h++) {
-                    const uint32_t k = k0 + h * 256u + tid;
-                    r[h] = 8u;  // no row
-                    if (k >= n) continue;
-                    const uint32_t tk = bt.task[k], lo = tk >> 3;
-                    r[h] = tk & 7u;
-                    const uint4 *q = p.big_rec + (size_t)(s0 + lo) * kBigRecQuads;
-                    c[h] = cktile[(s0 + lo) * ckstride + r[h]];
-                    x6[h] = q[0], x7[h] = q[1];
-                }
-#pragma unroll
-                for (uint32_t h = 0; h < 2u; h++) {
-                    if (r[h] >= 8u) continue;
-                    const uint32_t xs = g.X0 + (c[h].w & 0xffffu), xe = g.X0 + (c[h].w >> 16);
-                    TriRec R;
-                    R.dx[0] = __uint_as_float(x6[h].z), R.dx[1] = __uint_as_float(x6[h].w);
-                    R.dx[2] = __uint_as_float(x7[h].x);
-                    R.hz[0] = __uint_as_float(x7[h].y), R.hz[1] = __uint_as_float(x7[h].z);
-                    R.hz[2] = __uint_as_float(x7[h].w);
-                    R.flags = x6[h].y >> 30;
-                    const uint32_t pi0 = r[h] * (g.TW + 1u) + (xs - g.X0);
-                    const float w0 = __uint_as_float(c[h].x), w1 = __uint_as_float(c[h].y);
-                    const float w2 = __uint_as_float(c[h].z);
-                    if (mode == 1u) walk_span<1u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-                    else if (mode == 2u) walk_span<2u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-                    else walk_span<0u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-                }
+            for (uint32_t h = 0; h < 2u; h++) {
+                const uint32_t k = k0 + h * 256u + tid;
+                r[h] = 8u;  // no row
+                if (k >= total) continue;
+                const uint32_t tk = bt.task[k], tri = bt.tri[tk >> 3];
+                r[h] = tk & 7u;
+                const uint4 *q = p.big_rec + (size_t)tri * kBigRecQuads;
+                c[h] = cktile[tri * ckstride + r[h]];
+                x6[h] = q[0], x7[h] = q[1];
             }
-            __syncthreads();  // the next window reuses the task table
+#pragma unroll
+            for (uint32_t h = 0; h < 2u; h++) {
+                if (r[h] >= 8u) continue;
+                const uint32_t xs = g.X0 + (c[h].w & 0xffffu), xe = g.X0 + (c[h].w >> 16);
+                TriRec R;
+                R.dx[0] = __uint_as_float(x6[h].z), R.dx[1] = __uint_as_float(x6[h].w);
+                R.dx[2] = __uint_as_float(x7[h].x);
+                R.hz[0] = __uint_as_float(x7[h].y), R.hz[1] = __uint_as_float(x7[h].z);
+                R.hz[2] = __uint_as_float(x7[h].w);
+                R.flags = x6[h].y >> 30;
+                const uint32_t pi0 = r[h] * (g.TW + 1u) + (xs - g.X0);
+                const float w0 = __uint_as_float(c[h].x), w1 = __uint_as_float(c[h].y);
+                const float w2 = __uint_as_float(c[h].z);
+                if (mode == 1u) walk_span<1u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+                else if (mode == 2u) walk_span<2u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+                else walk_span<0u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+            }
         }
-        __syncthreads();  // the next round's prefix sum reuses the wave totals
+        __syncthreads();  // the next round reuses the table and the wave totals
     }
 }
 
@@ -1165,6 +1144,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         p.ctr->busy[npar][xcd][0] = 0u;
     }
     for (uint32_t i = blockIdx.x * 256u + tid; i < p.zero_n; i += gridDim.x * 256u) p.count_next[i] = TileBin{0u, 0u};
+    if constexpr (BIG)  // after a checkpoint frame: its list counts
+        for (uint32_t i = blockIdx.x * 256u + tid; i < p.ck_zero_n; i += gridDim.x * 256u) p.ck_count_next[i] = 0u;
     // The pool ran out in setup: entries were lost, the frame is dropped (the
     // host sees the flag and re-runs it with a larger pool).
     if (flags & FLAG_BIN_OVERFLOW) return;
@@ -1212,13 +1193,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         }
         const uint32_t cap = v.tile_cap;
         const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
-        // the tile's first 1024 checkpoint entries, loaded now (used after the bin)
-        unsigned long long e_first[4] = {0ull, 0ull, 0ull, 0ull};
+        // the tile's checkpoint list: its length and first 256 entries, loaded
+        // now (used after the bin; entries past the length are ignored)
+        uint32_t nck = 0u, e_first = 0u;
         if (BIG && ckp) {
-            const unsigned long long *trow = p.cktab + (size_t)tile * p.mstride;
-#pragma unroll
-            for (uint32_t i = 0; i < 4u; i++)
-                if (i * 256u + tid < nbig) e_first[i] = trow[i * 256u + tid];
+            nck = p.ck_count[tile];
+            if (tid < p.mstride) e_first = p.cktab[(size_t)tile * p.mstride + tid];
         }
         __syncthreads();
         // Runs of entries, kBatch at a time: this tile's bin and the big
@@ -1258,7 +1238,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             }
             raster_batch<!BATCH, BIG>(p, v, sh, tid, valid, prim, g, mode);
         }
-        if (BIG && ckp && nbig) big_tasks(p, sh, tid, tile, nbig, g, mode, e_first);
+        if (BIG && ckp) {
+            nck = __builtin_amdgcn_readfirstlane(nck);
+            if (nck) big_tasks(p, sh, tid, tile, nck, g, mode, e_first);
+        }
         uint32_t ntile = 0u;
         TileBin ntb{0u, 0u};
         if (BATCH && claim && tid == 0u && claimed < nbusy) {
@@ -1408,7 +1391,7 @@ static inline uint32_t blocks_of(size_t n, uint32_t b) { return (uint32_t)((n + 
 template <uint32_t SH>
 static void launch_setup_t(const FrameParams &p, hipStream_t s, bool batched, uint32_t blocks) {
     if (batched) {
-        hipLaunchKernelGGL((k_setup<SH, true>), dim3(blocks), dim3(256), 0, s, p);
+        hipLaunchKernelGGL((k_setup<SH, true, false>), dim3(blocks), dim3(256), 0, s, p);
     } else {
         const size_t n = std::max<size_t>(p.v0.n_tris, (size_t)p.v0.n_instances * p.v0.n_vertices);
         FrameParams q = p;
@@ -1421,7 +1404,10 @@ static void launch_setup_t(const FrameParams &p, hipStream_t s, bool batched, ui
             span_blocks = p.v0.n_tris * ((q.span_parts + 3u) / 4u);
             lds = 4u * 4u * (((kSpanRows >> sy) * p.v0.ntx + 3u) / 4u);
         }
-        hipLaunchKernelGGL((k_setup<SH, false>), dim3(q.setup_blocks + span_blocks), dim3(256), lds, s, q);
+        if (span_blocks)
+            hipLaunchKernelGGL((k_setup<SH, false, true>), dim3(q.setup_blocks + span_blocks), dim3(256), lds, s, q);
+        else
+            hipLaunchKernelGGL((k_setup<SH, false, false>), dim3(q.setup_blocks), dim3(256), 0, s, q);
     }
 }
 
@@ -1501,9 +1487,9 @@ static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, u
 // (triangles.rs:219-254, with the row's band), a bit in the wave's LDS mask
 // table for every tile column the span touches, then the walk from x0
 // (triangles.rs:257-301, the same w += dx additions as the raster's walk)
-// storing the weights where the span enters each column.  The wave then writes
-// the entries of its tiles.  Slots and entries sit at fixed places (layout in
-// internal.hpp), so no block waits for another.
+// storing the weights where the span enters each column.  The wave then appends
+// the entries of its tiles to their lists.  Slots sit at fixed places (layout
+// in internal.hpp), so no block waits for another.
 // ---------------------------------------------------------------------------
 EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b) {
     extern __shared__ uint32_t span_lds[];  // per wave: (kSpanRows >> sy) x ntx mask bytes
@@ -1574,12 +1560,13 @@ EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b) {
     __builtin_amdgcn_wave_barrier();
     // the entries of this item's tiles (every tile row of it is this item's)
     const uint32_t ty0 = ybase >> sy;
-    const unsigned long long tag = (unsigned long long)ck_tag(p.frame) << 40;
     for (uint32_t i = lane; i < groups * cols; i += 64u) {
         const uint32_t m = mb[i];
         if (m == 0u) continue;
         const uint32_t gi = i / cols, c = i - gi * cols;
-        p.cktab[((size_t)(ty0 + gi) * v.ntx + col0 + c) * p.mstride + t] = tag | ((unsigned long long)m << 32);
+        const size_t tile = (size_t)(ty0 + gi) * v.ntx + col0 + c;
+        const uint32_t k = atomicAdd(&p.ck_count[tile], 1u);  // below mstride: one entry per triangle
+        p.cktab[tile * p.mstride + k] = (t << 8) | m;
     }
 }
 

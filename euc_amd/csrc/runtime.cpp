@@ -70,7 +70,9 @@ struct euchip_ctx {
     DevArray<uint32_t> big;     // big-triangle list
     DevArray<uint4> big_rec;    // per triangle of a checkpoint frame: what its row tasks read
     DevArray<uint4> ck;         // row checkpoints of big triangles
-    DevArray<unsigned long long> cktab;  // their tagged row masks per (tile, triangle)
+    DevArray<uint32_t> cktab;   // per tile: its list of (triangle, row mask)
+    DevArray<uint32_t> ck_count;  // the lists' lengths, two halves like tile_count
+    uint32_t ck_dirty[2] = {0, 0};  // counts a checkpoint frame left in each half, zeroed by the next frame
     DevArray<uint32_t> lrec;    // line records
     DevArray<uint64_t> lkey;    // per-pixel winner keys of a lines frame
     std::vector<Deferred> deferred;
@@ -566,9 +568,12 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     // two halves: setup counts into one while the raster pass zeroes the other
     const bool fresh_tiles = ctx->tile_count.cap < 2 * ntiles;
     if ((s = ensure(ctx, ctx->tile_count, 2 * ntiles, "tile counts", ctx->stream))) return s;
-    if (fresh_tiles) {
+    if (fresh_tiles) {  // new halves: the checkpoint list counts start over with them
         HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * sizeof(TileBin), ctx->stream));
         ctx->counts_used[0] = ctx->counts_used[1] = 0;
+        if (ctx->ck_count.cap)
+            HIP_TRY(ctx, hipMemsetAsync(ctx->ck_count.ptr, 0, ctx->ck_count.cap * 4u, ctx->stream));
+        ctx->ck_dirty[0] = ctx->ck_dirty[1] = 0;
     }
     const size_t half = ctx->tile_count.cap / 2;
     if ((s = ensure(ctx, ctx->heavy, ntiles, "heavy tiles", ctx->stream))) return s;
@@ -586,12 +591,15 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     const size_t ck_slots = (size_t)P.n_tris * p.v0.ntx * p.v0.H;
     if (big_mode && p.tile_sx >= 5 && ntiles * mstride <= ((size_t)8 << 20) && ck_slots <= kCkMaxSlots &&
         4u * (kSpanRows >> (8u - p.tile_sx)) * (size_t)p.v0.ntx <= ((size_t)64 << 10)) {
-        const size_t mask_had = ctx->cktab.cap;
+        const size_t counts_had = ctx->ck_count.cap;
         ck_on = ensure(ctx, ctx->big_rec, (size_t)P.n_tris * kBigRecQuads, "big triangles", ctx->stream) == EUCHIP_OK &&
                 ensure(ctx, ctx->ck, ck_slots, "row checkpoints", ctx->stream) == EUCHIP_OK &&
-                ensure(ctx, ctx->cktab, ntiles * mstride, "row masks", ctx->stream) == EUCHIP_OK;
-        if (ck_on && ctx->cktab.cap != mask_had)
-            HIP_TRY(ctx, hipMemsetAsync(ctx->cktab.ptr, 0, ctx->cktab.cap * 8u, ctx->stream));
+                ensure(ctx, ctx->cktab, ntiles * mstride, "checkpoint lists", ctx->stream) == EUCHIP_OK &&
+                ensure(ctx, ctx->ck_count, ctx->tile_count.cap, "checkpoint lists", ctx->stream) == EUCHIP_OK;
+        if (ck_on && ctx->ck_count.cap != counts_had) {  // both halves start at zero
+            HIP_TRY(ctx, hipMemsetAsync(ctx->ck_count.ptr, 0, ctx->ck_count.cap * 4u, ctx->stream));
+            ctx->ck_dirty[0] = ctx->ck_dirty[1] = 0;
+        }
     }
 
     for (int attempt = 0;; attempt++) {
@@ -608,6 +616,20 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.tile_count = ctx->tile_count.ptr + par * half;
         p.count_next = ctx->tile_count.ptr + npar * half;
         p.zero_n = ctx->counts_used[npar];
+        // Checkpoint list counts: a checkpoint frame counts into its half, which
+        // must be zero.  The raster of a big-triangle frame zeroes the other half
+        // (left by the frame before); a half still left after other frames is
+        // cleared here first.
+        if (ck_on) {
+            p.ck_count = ctx->ck_count.ptr + par * half;
+            if (ctx->ck_dirty[par]) HIP_TRY(ctx, launch_fill_u32(p.ck_count, 0u, ctx->ck_dirty[par], ctx->stream));
+            ctx->ck_dirty[par] = 0;
+        }
+        p.ck_count_next = nullptr, p.ck_zero_n = 0u;
+        if (p.big_cap && ctx->ck_dirty[npar]) {
+            p.ck_count_next = ctx->ck_count.ptr + npar * half;
+            p.ck_zero_n = ctx->ck_dirty[npar];
+        }
         p.busy = ctx->busy.ptr;
         p.busy_map = BUSY_BLOCKS;
         p.big = ctx->big.ptr;
@@ -627,12 +649,12 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         p.heavy = ctx->heavy.ptr;
         p.heavy_cap = (uint32_t)std::min<size_t>(ctx->heavy.cap, 0xFFFFFFF0ull);
         p.frame = ctx->frame++;
-        if (p.ck_on && p.frame % 0xffffffu == 0u)  // the tags start over: no old entry may match
-            HIP_TRY(ctx, hipMemsetAsync(ctx->cktab.ptr, 0, ctx->cktab.cap * 8u, ctx->stream));
         if ((s = timed(ctx, "setup", [&] { return launch_setup(p, ctx->stream, false, 0); }))) return s;
         if ((s = timed(ctx, "raster", [&] { return launch_raster(p, ctx->stream, false, ctx->max_groups); }))) return s;
         ctx->counts_used[npar] = 0;
         ctx->counts_used[par] = (uint32_t)ntiles;
+        if (p.ck_zero_n) ctx->ck_dirty[npar] = 0;
+        if (ck_on) ctx->ck_dirty[par] = (uint32_t)ntiles;
         if (!sync) return EUCHIP_OK;
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->profiling) resolve_pending(ctx);
@@ -1010,6 +1032,7 @@ euchip_status euchip_ctx_destroy(euchip_ctx *ctx) {
     release(ctx, ctx->big_rec);
     release(ctx, ctx->ck);
     release(ctx, ctx->cktab);
+    release(ctx, ctx->ck_count);
     release(ctx, ctx->lrec);
     release(ctx, ctx->lkey);
 
