@@ -144,3 +144,31 @@ def test_checkpoints_msaa(gpu_ctx, level):
     r = run_pair(gpu_ctx, d, S.uniforms(), m, 1600, 1000)
     assert r["frags"] > 10 ** 5
     assert_bit_exact(r, f"corner triangles msaa {level}")
+
+
+def test_checkpoint_frames_among_other_frames():
+    # Checkpoint frames count their tiles' lists into the half of the count
+    # buffer their frame parity selects; the next big-triangle frame's raster
+    # clears the other half, and a half left over when other frames came between
+    # is cleared before it is counted into again.  Frames of every kind in one
+    # context, each bit-exact: checkpoint frames of both parities, binned frames
+    # between them (one, two, three in a row), and a larger target that starts
+    # the count buffers over.
+    from euc_amd.renderer import Context
+    plain = S.plain_scene()
+    binned = S.icon_request(0x8A1, min_tris=6000, max_tris=8000)  # more triangles than tiles: binned
+    corner = corner_triangles(23, 20)
+    seq = ["ck", "bin", "ck", "ck", "bin", "bin", "ck", "bin", "bin", "bin", "ck", "ck", "big", "ck", "bin", "ck"]
+    ctx = Context(0)
+    try:
+        for i, kind in enumerate(seq):
+            if kind == "ck":
+                r = run_pair(ctx, flat(), plain.uniforms, plain.mesh, 640, 480, color_init=0x11223344)
+            elif kind == "big":
+                r = run_pair(ctx, flat(), S.uniforms(), corner, 2048, 2048, color_init=0xFF000000)
+            else:
+                d = PipelineDesc.make(Shader.PHONG, Xform.MVP, depth_mode=DepthMode.LESS_WRITE, cull=CullMode.BACK)
+                r = run_pair(ctx, d, binned.uniforms, binned.mesh, binned.width, binned.height)
+            assert_bit_exact(r, f"frame {i} ({kind})")
+    finally:
+        ctx.close()
