@@ -626,7 +626,7 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
             ctx->ck_dirty[par] = 0;
         }
         p.ck_count_next = nullptr, p.ck_zero_n = 0u;
-        if (p.big_cap && ctx->ck_dirty[npar]) {
+        if (big_mode && ctx->ck_dirty[npar]) {  // the raster is the big-triangle kernel
             p.ck_count_next = ctx->ck_count.ptr + npar * half;
             p.ck_zero_n = ctx->ck_dirty[npar];
         }
