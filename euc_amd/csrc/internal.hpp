@@ -205,9 +205,11 @@ struct FrameParams {
     // (ntiles x mstride entries, mstride >= n_tris) with their counts (two
     // halves by frame parity like tile_count: ck_count this frame's,
     // ck_count_next the other, whose first ck_zero_n counts a big-triangle
-    // frame's raster zeroes);
-    // the setup launch then has setup_blocks setup blocks, then the span
-    // blocks (span_parts items of kSpanRows rows per triangle, 4 per block).
+    // frame's raster zeroes).
+    // A one-request setup launch has tri_blocks triangle blocks, then vertex
+    // blocks up to setup_blocks; a checkpoint frame's has setup_blocks blocks
+    // of triangles and vertices, then the span blocks (span_parts items of
+    // kSpanRows rows per triangle, 4 per block).
     uint4 *big_rec;
     uint4 *ck;
     uint32_t ck_on;
@@ -215,7 +217,7 @@ struct FrameParams {
     uint32_t mstride;
     uint32_t *ck_count, *ck_count_next;
     uint32_t ck_zero_n;
-    uint32_t setup_blocks, span_parts;
+    uint32_t tri_blocks, setup_blocks, span_parts;
     TileBin *count_next;
     uint32_t zero_n;
     uint32_t frame;        // frame parity selects the overflow counter and tile queue

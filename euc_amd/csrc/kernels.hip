@@ -487,13 +487,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     __shared__ SetupShared sm;
     __shared__ uint32_t npl;
     __shared__ uint32_t lcnt[9], lbase[9];  // new busy tiles per list (8) and new heavy tiles (8)
-    if constexpr (SPAN) {
-        if (blockIdx.x >= p.setup_blocks) {
-            span_item(p, blockIdx.x - p.setup_blocks);
+    const uint32_t tid = threadIdx.x;
+    if constexpr (!BATCH) {
+        if (blockIdx.x >= p.tri_blocks) {  // (checkpoint frames: tri_blocks = setup_blocks)
+            if constexpr (SPAN) {
+                span_item(p, blockIdx.x - p.setup_blocks);
+            } else if constexpr (vd_count(SH) > 0) {  // a vertex block
+                const View &v = p.v0;
+                const uint32_t t = (blockIdx.x - p.tri_blocks) * 256u + tid;
+                if (t < v.n_instances * v.n_vertices) {
+                    const uint32_t inst = t / v.n_vertices;
+                    float vd[VDN<SH>::N];
+                    vertex_data<SH>(p, v, inst, t - inst * v.n_vertices, vd);
+#pragma unroll
+                    for (uint32_t q = 0; q < vd_count(SH); q++) p.vary[((size_t)v.sv_base + t) * vd_count(SH) + q] = vd[q];
+                }
+            }
             return;
         }
     }
-    const uint32_t tid = threadIdx.x;
     uint32_t r = 0, j = blockIdx.x;
     if constexpr (BATCH) {
         const uint2 bj = p.blk[blockIdx.x];
@@ -539,9 +551,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         }
     }
     sm.rec.kept[tid] = keep ? 1u : 0u;
-    // The vertex stage after the triangles' loads: the VertexData stores would
-    // otherwise order those loads behind them.
-    if constexpr (vd_count(SH) > 0) {
+    // Batches and checkpoint frames: the vertex stage after the triangles' loads
+    // (the VertexData stores would otherwise order those loads behind them);
+    // other one-request frames have vertex blocks of their own.
+    if constexpr ((BATCH || SPAN) && vd_count(SH) > 0) {
         if (t < v.n_instances * v.n_vertices) {
             const uint32_t inst = t / v.n_vertices;
             const uint32_t vi = t - inst * v.n_vertices;
@@ -1393,9 +1406,9 @@ static void launch_setup_t(const FrameParams &p, hipStream_t s, bool batched, ui
     if (batched) {
         hipLaunchKernelGGL((k_setup<SH, true, false>), dim3(blocks), dim3(256), 0, s, p);
     } else {
-        const size_t n = std::max<size_t>(p.v0.n_tris, (size_t)p.v0.n_instances * p.v0.n_vertices);
+        // triangle blocks, then vertex blocks; checkpoint frames: blocks of
+        // triangles and vertices, then span blocks
         FrameParams q = p;
-        q.setup_blocks = blocks_of(n, 256);
         uint32_t span_blocks = 0;
         size_t lds = 0;
         if (p.ck_on && p.v0.H) {  // span blocks: 4 items of kSpanRows rows per triangle and block
@@ -1403,6 +1416,13 @@ static void launch_setup_t(const FrameParams &p, hipStream_t s, bool batched, ui
             q.span_parts = (p.v0.H + (1u << sy) + kSpanRows - 1u) / kSpanRows;
             span_blocks = p.v0.n_tris * ((q.span_parts + 3u) / 4u);
             lds = 4u * 4u * (((kSpanRows >> sy) * p.v0.ntx + 3u) / 4u);
+        }
+        const size_t nv = (size_t)p.v0.n_instances * p.v0.n_vertices;
+        if (span_blocks) {
+            q.tri_blocks = q.setup_blocks = blocks_of(std::max<size_t>(p.v0.n_tris, nv), 256);
+        } else {
+            q.tri_blocks = blocks_of(p.v0.n_tris, 256);
+            q.setup_blocks = q.tri_blocks + (vd_count(SH) > 0 ? blocks_of(nv, 256) : 0u);
         }
         if (span_blocks)
             hipLaunchKernelGGL((k_setup<SH, false, true>), dim3(q.setup_blocks + span_blocks), dim3(256), lds, s, q);
