@@ -245,8 +245,8 @@ euchip_status euchip_batch_destroy(euchip_batch *b);
  * EUCHIP_BAND_ZERO, EUCHIP_INVALID_ARG) naming its index in last_error. */
 euchip_status euchip_batch_set(euchip_batch *b, const euchip_pipeline_desc *desc, const euchip_request *reqs,
                                uint32_t n);
-/* Replace the requests, request i rendered with descs[i] (the per-request desc
- * array SURVEY's batch entry point names; no counterpart in euc).  Requests of
+/* Replace the requests, request i rendered with descs[i] (no counterpart in
+ * euc: a batch of render() calls of different pipelines).  Requests of
  * identical descs share one setup and one raster launch; each distinct desc
  * runs as a part of the batch on a stream of its own, so the parts overlap.
  * launch / wait / query / set_tile_width / autotune act on every part
