@@ -124,6 +124,8 @@ SIGNATURES = {
                                        C.POINTER(C.c_void_p)]),
     "euchip_server_destroy": (C.c_int, [C.c_void_p]),
     "euchip_server_submit": (C.c_int, [C.c_void_p, C.POINTER(ServeRequestC), C.POINTER(C.c_uint64)]),
+    "euchip_server_submit_desc": (C.c_int, [C.c_void_p, C.POINTER(ServeRequestC), C.POINTER(PipelineDescC),
+                                            C.POINTER(C.c_uint64)]),
     "euchip_server_submit_n": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "euchip_server_poll": (C.c_int, [C.c_void_p, C.POINTER(CompletionC), C.c_uint32, C.POINTER(C.c_uint32)]),
     "euchip_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),

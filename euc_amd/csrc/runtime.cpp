@@ -1409,8 +1409,8 @@ euchip_status euchip_batch_set_multi(euchip_batch *b, const euchip_pipeline_desc
     euchip_ctx *ctx = b->ctx;
     (void)hipSetDevice(ctx->device);
     HIP_TRY(ctx, hipStreamSynchronize(b->stream));
-    drop_parts(b);
-    // this batch's own requests become empty; the parts replace them
+    // this batch's own requests become empty; the parts replace them (existing
+    // parts are reused, with the storage they grew)
     b->reqs.clear();
     b->uniforms.clear();
     b->plans.clear();
@@ -1427,11 +1427,18 @@ euchip_status euchip_batch_set_multi(euchip_batch *b, const euchip_pipeline_desc
         if (g == kinds.size()) kinds.push_back(descs[i]), members.emplace_back();
         members[g].push_back(i);
     }
+    while (b->parts.size() > kinds.size()) {
+        euchip_batch_destroy(b->parts.back());
+        b->parts.pop_back();
+    }
     for (size_t g = 0; g < kinds.size(); g++) {
-        euchip_batch *q = nullptr;
-        if ((s = euchip_batch_create(ctx, &q))) break;
-        q->tile_sx = b->tile_sx;
-        b->parts.push_back(q);
+        if (g == b->parts.size()) {
+            euchip_batch *q = nullptr;
+            if ((s = euchip_batch_create(ctx, &q))) break;
+            q->tile_sx = b->tile_sx;
+            b->parts.push_back(q);
+        }
+        euchip_batch *q = b->parts[g];
         std::vector<euchip_request> rs;
         for (uint32_t i : members[g]) rs.push_back(reqs[i]);
         if ((s = euchip_batch_set(q, &kinds[g], rs.data(), (uint32_t)rs.size()))) {

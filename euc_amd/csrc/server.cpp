@@ -42,6 +42,8 @@ struct Queued {
     const euchip_buffer *texture, *shadow;
     std::array<float, EUCHIP_UNIFORM_FLOATS> u;
     uint32_t w, h;
+    bool own = false;             // rendered with `desc` instead of the server's
+    euchip_pipeline_desc desc{};
 };
 
 struct Slot {
@@ -229,7 +231,7 @@ bool admit(euchip_server *s, Slot &sl, bool &freed) {
         const Queued &q = s->queue.front();
         Target t;
         const bool pooled = take_pooled(s, q.w, q.h, t);
-        const size_t sc = scratch + euchip::request_scratch_estimate(&s->desc, q.mesh, q.w, q.h);
+        const size_t sc = scratch + euchip::request_scratch_estimate(q.own ? &q.desc : &s->desc, q.mesh, q.w, q.h);
         const size_t add = pooled ? 0 : target_bytes(q.w, q.h);
         const size_t need = new_bytes + add + need_of(sc);
         if (s->budget && !make_room(s, need)) {
@@ -260,7 +262,15 @@ bool admit(euchip_server *s, Slot &sl, bool &freed) {
             reqs[i].texture = take[i].texture;
             reqs[i].shadow = take[i].shadow;
         }
-        st = euchip_batch_set(sl.batch, &s->desc, reqs.data(), (uint32_t)reqs.size());
+        bool own = false;
+        for (const Queued &q : take) own = own || q.own;
+        if (!own) {
+            st = euchip_batch_set(sl.batch, &s->desc, reqs.data(), (uint32_t)reqs.size());
+        } else {  // requests with pipelines of their own: one part per distinct desc
+            std::vector<euchip_pipeline_desc> descs(take.size());
+            for (size_t i = 0; i < take.size(); i++) descs[i] = take[i].own ? take[i].desc : s->desc;
+            st = euchip_batch_set_multi(sl.batch, descs.data(), reqs.data(), (uint32_t)reqs.size());
+        }
     }
     if (!st) st = euchip_batch_launch(sl.batch, 0);
     if (st == EUCHIP_OVER_BUDGET || st == EUCHIP_OOM) {
@@ -290,17 +300,10 @@ bool admit(euchip_server *s, Slot &sl, bool &freed) {
     return true;
 }
 
-}  // namespace
-
-extern "C" {
-
-euchip_status euchip_server_create(euchip_ctx *ctx, const euchip_pipeline_desc *desc, size_t budget_bytes,
-                                   uint32_t max_batch, uint32_t slots, euchip_server **out) {
-    if (!ctx || !desc || !out || max_batch == 0 || slots == 0) return EUCHIP_INVALID_ARG;
-    *out = nullptr;
-    // The server owns its targets and hands out pooled storage that still holds
-    // an earlier request's pixels and depth: every request must start from the
-    // desc's clear values, or it would see (and depth-test against) another's.
+// The server owns its targets and hands out pooled storage that still holds an
+// earlier request's pixels and depth: every request must start from its desc's
+// clear values, or it would see (and depth-test against) another's.
+euchip_status check_server_desc(euchip_ctx *ctx, const euchip_pipeline_desc *desc) {
     if (desc->primitives != EUCHIP_PRIMS_TRIANGLE_LIST) {
         euchip::set_last_error(ctx, "servers draw triangle lists");
         return EUCHIP_INVALID_ARG;
@@ -310,6 +313,18 @@ euchip_status euchip_server_create(euchip_ctx *ctx, const euchip_pipeline_desc *
                                    "pooled targets hold earlier requests' results");
         return EUCHIP_INVALID_ARG;
     }
+    return EUCHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+euchip_status euchip_server_create(euchip_ctx *ctx, const euchip_pipeline_desc *desc, size_t budget_bytes,
+                                   uint32_t max_batch, uint32_t slots, euchip_server **out) {
+    if (!ctx || !desc || !out || max_batch == 0 || slots == 0) return EUCHIP_INVALID_ARG;
+    *out = nullptr;
+    if (const euchip_status st = check_server_desc(ctx, desc)) return st;
     euchip_server *s = new euchip_server();
     s->ctx = ctx;
     s->desc = *desc;
@@ -344,9 +359,14 @@ euchip_status euchip_server_destroy(euchip_server *s) {
     return EUCHIP_OK;
 }
 
-euchip_status euchip_server_submit(euchip_server *s, const euchip_serve_request *r, uint64_t *ticket) {
+euchip_status euchip_server_submit_desc(euchip_server *s, const euchip_serve_request *r,
+                                       const euchip_pipeline_desc *desc, uint64_t *ticket) {
     if (!s || !r || !r->mesh || !r->uniforms || !r->width || !r->height) return EUCHIP_INVALID_ARG;
+    if (desc) {
+        if (const euchip_status st = check_server_desc(s->ctx, desc)) return st;
+    }
     Queued q;
+    if (desc) q.own = true, q.desc = *desc;
     q.ticket = s->next_ticket++;
     q.mesh = r->mesh;
     q.texture = r->texture;
@@ -358,6 +378,10 @@ euchip_status euchip_server_submit(euchip_server *s, const euchip_serve_request 
     s->st.submitted++;
     if (ticket) *ticket = q.ticket;
     return EUCHIP_OK;
+}
+
+euchip_status euchip_server_submit(euchip_server *s, const euchip_serve_request *r, uint64_t *ticket) {
+    return euchip_server_submit_desc(s, r, nullptr, ticket);
 }
 
 euchip_status euchip_server_submit_n(euchip_server *s, const euchip_serve_request *reqs, uint32_t n,

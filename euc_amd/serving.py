@@ -61,14 +61,23 @@ class Server:
         ctx._children.add(self)
 
     def submit(self, mesh: GpuMesh, uniforms: np.ndarray, width: int, height: int,
-               texture: Optional[Buffer2d] = None, shadow: Optional[Buffer2d] = None) -> int:
+               texture: Optional[Buffer2d] = None, shadow: Optional[Buffer2d] = None,
+               desc: Optional[PipelineDesc] = None, clear_color: Optional[int] = 0,
+               clear_depth: Optional[float] = 1.0) -> int:
+        """Queue one request; `desc` (with its clears) renders it with a pipeline
+        of its own instead of the server's."""
         u = np.ascontiguousarray(uniforms, dtype=np.float32)
         assert u.size == N_UNIFORMS
         r = ServeRequestC(mesh=mesh.handle, texture=texture.handle if texture else None,
                           shadow=shadow.handle if shadow else None, uniforms=u.ctypes.data,
                           width=int(width), height=int(height))
         t = C.c_uint64()
-        _check(self.ctx.handle, _lib.lib().euchip_server_submit(self.handle, C.byref(r), C.byref(t)))
+        if desc is None:
+            _check(self.ctx.handle, _lib.lib().euchip_server_submit(self.handle, C.byref(r), C.byref(t)))
+        else:
+            d = desc_c(desc, None, None, clear_color, clear_depth)
+            _check(self.ctx.handle, _lib.lib().euchip_server_submit_desc(self.handle, C.byref(r), C.byref(d),
+                                                                          C.byref(t)))
         self.sizes[t.value] = (int(width), int(height))
         return t.value
 

@@ -315,6 +315,12 @@ euchip_status euchip_server_create(euchip_ctx *ctx, const euchip_pipeline_desc *
                                    uint32_t max_batch, uint32_t slots, euchip_server **out);
 euchip_status euchip_server_destroy(euchip_server *srv);
 euchip_status euchip_server_submit(euchip_server *srv, const euchip_serve_request *req, uint64_t *ticket);
+/* Submit a request rendered with its own pipeline desc (NULL: the server's),
+ * which must draw triangle lists with both clears enabled (EUCHIP_INVALID_ARG
+ * otherwise).  Requests of different descs admitted together run as parts of
+ * one batch (euchip_batch_set_multi).  No counterpart in euc. */
+euchip_status euchip_server_submit_desc(euchip_server *srv, const euchip_serve_request *req,
+                                        const euchip_pipeline_desc *desc, uint64_t *ticket);
 /* Submit n requests in order; tickets (may be NULL) receives their tickets. */
 euchip_status euchip_server_submit_n(euchip_server *srv, const euchip_serve_request *reqs, uint32_t n,
                                      uint64_t *tickets);

@@ -112,3 +112,44 @@ def test_serving_rejects_what_cannot_fit():
     finally:
         ctx.set_budget(0)
         ctx.close()
+
+
+def test_serving_per_request_descs():
+    # Requests with pipelines of their own beside the server's: every result equals
+    # the request rendered alone with its pipeline, whatever shared its batch.
+    import time
+    ctx, items, _ = setup_ctx(30, max_tris=12000)
+    own = [None,
+           PipelineDesc.make(Shader.PHONG, Xform.MVP, depth_mode=DepthMode.LESS_WRITE, cull=CullMode.BACK),
+           PipelineDesc.make(Shader.GOURAUD, Xform.MVP, depth_mode=DepthMode.LESS_WRITE, cull=CullMode.NONE),
+           PipelineDesc.make(Shader.TEXTURED, Xform.MVP, depth_mode=DepthMode.GREATER_WRITE, cull=CullMode.BACK,
+                             band_policy=BandPolicy.SEQ)]
+    clears = [1.0, 1.0, 1.0, 0.0]  # GREATER_WRITE starts from depth 0
+    try:
+        picks = [i % len(own) for i in range(len(items))]
+        ref = []
+        for it, k in zip(items, picks):
+            c, d = ctx.color_target(it.width, it.height), ctx.depth_target(it.width, it.height)
+            ctx.render(own[k] or DESC, it.uniforms, it.mesh, c, d, texture=it.texture, clear_color=0,
+                       clear_depth=clears[k])
+            ref.append((c.download(), d.download()))
+            c.close(), d.close()
+        srv = Server(ctx, DESC, budget_bytes=0, max_batch=12, slots=2)
+        tickets = [srv.submit(it.mesh, it.uniforms, it.width, it.height, texture=it.texture, desc=own[k],
+                              clear_color=0, clear_depth=clears[k]) for it, k in zip(items, picks)]
+        got, t0 = {}, time.time()
+        while len(got) < len(tickets):
+            for cpl in srv.poll():
+                assert cpl.status == 0
+                got[cpl.ticket] = (cpl.pixel.download(), cpl.depth.download())
+                srv.release(cpl.ticket)
+            assert time.time() - t0 < 60, "requests still pending"
+        assert srv.stats()["batches"] >= 3
+        for i, t in enumerate(tickets):
+            assert_same(got[t][0], got[t][1], ref[i][0], ref[i][1], f"request {i} ({own[picks[i]] is not None})")
+        bad = PipelineDesc.make(Shader.PHONG, Xform.MVP)
+        with pytest.raises(Exception):  # no clears: pooled targets would leak earlier results
+            srv.submit(items[0].mesh, items[0].uniforms, 8, 8, desc=bad, clear_color=None, clear_depth=None)
+        srv.close()
+    finally:
+        ctx.close()
