@@ -1129,7 +1129,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     const uint32_t flags = p.ctr->flags[par][0];
     const uint32_t pool_used0 = p.ctr->pool[par][0];
     uint32_t nh = __builtin_amdgcn_readfirstlane(p.ctr->heavy[par][0]);
-    // big triangles: with checkpoints every triangle has a (tagged) entry per tile
+    // big triangles (with checkpoints any triangle may be on any tile's list)
     uint32_t nbig = !BIG ? 0u : p.ck_on ? p.v0.n_tris : __builtin_amdgcn_readfirstlane(p.ctr->big[par][0]);
     uint32_t nbusy = __builtin_amdgcn_readfirstlane(p.ctr->busy[par][xcd][0]);
     const uint32_t first = w < lcap ? __builtin_amdgcn_readfirstlane(list[w]) : 0u;
