@@ -1513,12 +1513,19 @@ static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, u
 // ---------------------------------------------------------------------------
 EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b) {
     extern __shared__ uint32_t span_lds[];  // per wave: (kSpanRows >> sy) x ntx mask bytes
+    __shared__ TriRec span_tri;              // the block's triangle, set up once by wave 0
+    __shared__ uint32_t span_keep;
     const View &v = p.v0;
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t t = b % v.n_tris, part = (b / v.n_tris) * 4u + wv;
-    if (part >= p.span_parts) return;
-    TriRec R;
-    if (!tri_setup(p, v, t, R)) return;
+    if (wv == 0u) {
+        TriRec R0;
+        const bool keep = tri_setup(p, v, t, R0);
+        if (lane == 0u) span_tri = R0, span_keep = keep ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!span_keep || part >= p.span_parts) return;
+    const TriRec R = span_tri;
     const uint32_t sx = p.tile_sx, sy = 8u - sx, TH = 1u << sy;
     const uint32_t col0 = R.xmin >> sx;
     const uint32_t cols = ((R.xmax - 1u) >> sx) - col0 + 1u;
