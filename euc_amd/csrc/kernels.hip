@@ -844,38 +844,48 @@ EUCHIP_DEV void walk_row(const FrameParams &p, const TriRec &R, uint32_t prim, u
     }
 }
 
-// walk_row without branches on the coverage: coverage, z, z clip and the key
-// of every pixel (z is only used where the pixel is covered), one predicated
-// LDS atomic.  The write modes' loop holds no LDS read, so nothing waits on the
-// atomics of earlier pixels.
+// The write modes' row loop: coverage, z, z clip and the key of every pixel,
+// then one LDS atomic min per pixel, with no branch: an uncovered pixel's
+// atomic takes the all-ones key, which changes nothing.  The loop holds no LDS
+// read, so nothing waits on the atomics of earlier pixels.
+//   * A NaN weight makes z NaN (a NaN factor or term stays NaN), so z == z and
+//     the minimum of the three weights >= 0 decide exactly what the three
+//     tests w >= 0 and z == z decide.
+//   * -0 + 0 = +0 gives both zeros one order value; z + 0 differs from z only
+//     for z = -0 (NaN z never wins), in the sign bit alone, so that bit of
+//     z ^ (z + 0) shifted in beside (prim + 1) keeps the sign in the key.
+template <uint32_t MODE, bool CLIP>
+EUCHIP_DEV void walk_keys(const TriRec &R, bool clip, float zlo, float zhi, uint32_t pid, unsigned long long *kp,
+                          unsigned long long *const ke, float w0, float w1, float w2) {
+    for (; kp < ke; kp++) {
+        const float u2 = (w2 - w0) - w1;
+        const float z = (R.hz[0] * w0 + R.hz[1] * w1) + R.hz[2] * u2;
+        bool ok = (__builtin_fminf(__builtin_fminf(w0, w1), u2) >= 0.0f) & (z == z);
+        if constexpr (CLIP) ok = ok & (!clip | ((zlo <= z) & (z <= zhi)));
+        const uint32_t bz = __float_as_uint(z), u = __float_as_uint(z + 0.0f);
+        const uint32_t o = u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
+        const uint32_t lo = __builtin_amdgcn_alignbit(pid, bz ^ u, 31u);  // (pid << 1) | (z is -0)
+        const uint64_t k = ((uint64_t)(MODE == 1u ? o : ~o) << 32) | lo;
+        atomicMin(kp, ok ? k : ~0ull);
+        w0 = w0 + R.dx[0];
+        w1 = w1 + R.dx[1];
+        w2 = w2 + R.dx[2];
+    }
+}
+
+// walk_row without branches on the coverage; the write modes take walk_keys,
+// with the z clip only where a lane of the wave needs it.
 template <uint32_t MODE>
 EUCHIP_DEV void walk_span(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t xs, uint32_t xe, uint32_t pi0,
                           float w0, float w1, float w2, uint64_t *key, const float *d0s) {
     const bool clip = !(R.flags & TRI_ALL_IN) && p.z_clip;
     if constexpr (MODE != 0u) {
-        // Write modes: a NaN z never wins.  -0 + 0 = +0 gives both zeros one
-        // order value; the key's low bit keeps the sign.  Rows of triangles
-        // that need no z clip (every lane of the wave) take the loop without it.
-        const uint32_t lo0 = (prim + 1u) << 1, lo1 = lo0 | 1u;
         unsigned long long *kp = (unsigned long long *)(key + pi0);
         unsigned long long *const ke = kp + (xe - xs);
-        const bool any_clip = __builtin_amdgcn_ballot_w64(clip) != 0ull;
-        const float zlo = p.z_min, zhi = p.z_max;
-        for (; kp < ke; kp++) {
-            const float u2 = (w2 - w0) - w1;
-            const float z = (R.hz[0] * w0 + R.hz[1] * w1) + R.hz[2] * u2;
-            bool ok = (w0 >= 0.0f) & (w1 >= 0.0f) & (u2 >= 0.0f) & (z == z);
-            if (any_clip) ok = ok & (!clip | ((zlo <= z) & (z <= zhi)));
-            if (ok) {
-                const uint32_t u = __float_as_uint(z + 0.0f);
-                const uint32_t o = u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
-                const uint32_t lo = __float_as_uint(z) == 0x80000000u ? lo1 : lo0;
-                atomicMin(kp, ((uint64_t)(MODE == 1u ? o : ~o) << 32) | lo);
-            }
-            w0 = w0 + R.dx[0];
-            w1 = w1 + R.dx[1];
-            w2 = w2 + R.dx[2];
-        }
+        if (__builtin_amdgcn_ballot_w64(clip) != 0ull)
+            walk_keys<MODE, true>(R, clip, p.z_min, p.z_max, prim + 1u, kp, ke, w0, w1, w2);
+        else
+            walk_keys<MODE, false>(R, clip, p.z_min, p.z_max, prim + 1u, kp, ke, w0, w1, w2);
         return;
     }
     const float zlo = p.z_min, zhi = p.z_max;
