@@ -854,10 +854,15 @@ EUCHIP_DEV void walk_row(const FrameParams &p, const TriRec &R, uint32_t prim, u
 //   * -0 + 0 = +0 gives both zeros one order value; z + 0 differs from z only
 //     for z = -0 (NaN z never wins), in the sign bit alone, so that bit of
 //     z ^ (z + 0) shifted in beside (prim + 1) keeps the sign in the key.
-template <uint32_t MODE, bool CLIP>
+//   * An uncovered pixel's key has an all-ones high word: no stored key has
+//     one (order values of non-NaN depths stay below it), so it never wins.
+//   * PAIRS: two pixels per turn (the second atomic at the next address)
+//     halve the loop control; a row of odd length ends with one more pixel.
+//     It pays for long rows only.
+template <uint32_t MODE, bool CLIP, bool PAIRS>
 EUCHIP_DEV void walk_keys(const TriRec &R, bool clip, float zlo, float zhi, uint32_t pid, unsigned long long *kp,
                           unsigned long long *const ke, float w0, float w1, float w2) {
-    for (; kp < ke; kp++) {
+    auto pixel = [&](unsigned long long *at) {
         const float u2 = (w2 - w0) - w1;
         const float z = (R.hz[0] * w0 + R.hz[1] * w1) + R.hz[2] * u2;
         bool ok = (__builtin_fminf(__builtin_fminf(w0, w1), u2) >= 0.0f) & (z == z);
@@ -865,17 +870,28 @@ EUCHIP_DEV void walk_keys(const TriRec &R, bool clip, float zlo, float zhi, uint
         const uint32_t bz = __float_as_uint(z), u = __float_as_uint(z + 0.0f);
         const uint32_t o = u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
         const uint32_t lo = __builtin_amdgcn_alignbit(pid, bz ^ u, 31u);  // (pid << 1) | (z is -0)
-        const uint64_t k = ((uint64_t)(MODE == 1u ? o : ~o) << 32) | lo;
-        atomicMin(kp, ok ? k : ~0ull);
+        const uint32_t hi = ok ? (MODE == 1u ? o : ~o) : ~0u;
+        atomicMin(at, ((uint64_t)hi << 32) | lo);
         w0 = w0 + R.dx[0];
         w1 = w1 + R.dx[1];
         w2 = w2 + R.dx[2];
+    };
+    if constexpr (PAIRS) {
+        const uint32_t n = (uint32_t)(ke - kp);
+        unsigned long long *const pairs_end = kp + (n & ~1u);
+        for (; kp < pairs_end; kp += 2) {
+            pixel(kp);
+            pixel(kp + 1);
+        }
+        if (n & 1u) pixel(kp);
+    } else {
+        for (; kp < ke; kp++) pixel(kp);
     }
 }
 
 // walk_row without branches on the coverage; the write modes take walk_keys,
 // with the z clip only where a lane of the wave needs it.
-template <uint32_t MODE>
+template <uint32_t MODE, bool PAIRS = false>
 EUCHIP_DEV void walk_span(const FrameParams &p, const TriRec &R, uint32_t prim, uint32_t xs, uint32_t xe, uint32_t pi0,
                           float w0, float w1, float w2, uint64_t *key, const float *d0s) {
     const bool clip = !(R.flags & TRI_ALL_IN) && p.z_clip;
@@ -883,9 +899,9 @@ EUCHIP_DEV void walk_span(const FrameParams &p, const TriRec &R, uint32_t prim, 
         unsigned long long *kp = (unsigned long long *)(key + pi0);
         unsigned long long *const ke = kp + (xe - xs);
         if (__builtin_amdgcn_ballot_w64(clip) != 0ull)
-            walk_keys<MODE, true>(R, clip, p.z_min, p.z_max, prim + 1u, kp, ke, w0, w1, w2);
+            walk_keys<MODE, true, PAIRS>(R, clip, p.z_min, p.z_max, prim + 1u, kp, ke, w0, w1, w2);
         else
-            walk_keys<MODE, false>(R, clip, p.z_min, p.z_max, prim + 1u, kp, ke, w0, w1, w2);
+            walk_keys<MODE, false, PAIRS>(R, clip, p.z_min, p.z_max, prim + 1u, kp, ke, w0, w1, w2);
         return;
     }
     const float zlo = p.z_min, zhi = p.z_max;
@@ -939,8 +955,8 @@ EUCHIP_DEV void raster_row(const FrameParams &p, const TriRec &R, uint32_t prim,
         w2 = w2 + R.dx[2];
     }
     const uint32_t pi0 = r * (g.TW + 1u) + (xs - g.X0);
-    if (mode == 1u) walk_span<1u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
-    else if (mode == 2u) walk_span<2u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
+    if (mode == 1u) walk_span<1u, (SKIP_UNROLL > 4u)>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
+    else if (mode == 2u) walk_span<2u, (SKIP_UNROLL > 4u)>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
     else walk_row<0u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
 }
 
@@ -1103,8 +1119,8 @@ EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, ui
                 const uint32_t pi0 = r[h] * (g.TW + 1u) + (xs - g.X0);
                 const float w0 = __uint_as_float(c[h].x), w1 = __uint_as_float(c[h].y);
                 const float w2 = __uint_as_float(c[h].z);
-                if (mode == 1u) walk_span<1u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
-                else if (mode == 2u) walk_span<2u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+                if (mode == 1u) walk_span<1u, true>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
+                else if (mode == 2u) walk_span<2u, true>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
                 else walk_span<0u>(p, R, x6[h].x, xs, xe, pi0, w0, w1, w2, sh.key, sh.d0s);
             }
         }
