@@ -115,11 +115,26 @@ constexpr uint32_t kBigCap = 4096;
 // + y of the pool (one 16-byte slot per triangle, column and row of the
 // target, so the 8 rows of one tile are consecutive): (w0, w1, w2) at xs, then
 // (xs - X0) | (xe - X0) << 16, the row's span inside the column that starts at
-// X0.  Each tile some row of which has such a span gets the 4-byte entry
-// t << 8 | (mask of those rows) in its list (cktab, tile * mstride + k, k from
-// the tile's count in ck_count).  The raster pass reads its tile's list and
-// walks those rows from their checkpoints.  Tiles are then at most 8 rows high.
+// X0.  Each tile some row of which has such a span gets the 8-byte entry
+// (t << 8 | mask of those rows, klo) in its list (cktab, tile * mstride + k, k
+// from the tile's count, ck_count[tile].x).  The raster pass reads its tile's
+// list and walks those rows from their checkpoints.  Tiles are then at most 8
+// rows high.
+// Occlusion (write modes, Less or Greater with depth write, where the winner is
+// the fragment of least key): klo is a lower bound of the key order (zord of z,
+// or ~zord) of every fragment the triangle has in the tile.  A triangle whose
+// spans cover every pixel of the tile, with margins that hold under the walk's
+// roundings, raises ck_count[tile].y to ~khi, khi an upper bound of its keys'
+// order there.  An entry with klo above the tile's least khi can win no pixel:
+// the raster skips it.  (Bounds: the triangle's weight and depth planes at the
+// tile's corners, widened by what the row walk's roundings can add; the
+// derivation is at tile_bounds in kernels.hip.)
 constexpr uint32_t kSpanRows = 64;             // rows per item of the checkpoint walk (one wave)
+// Bytes of one span wave's LDS tables: two row-mask bytes per (tile row of
+// the item, tile column).
+constexpr uint32_t span_table_bytes(uint32_t sy, uint32_t ntx) {
+    return 2u * ((((kSpanRows >> sy) * ntx) + 3u) & ~3u);
+}
 // Per triangle of a checkpoint frame (big_rec): (primitive, flags << 30, dx0,
 // dx1), (dx2, hz0, hz1, hz2), what the raster's row tasks read besides the slot.
 constexpr uint32_t kBigRecQuads = 2;
@@ -213,9 +228,9 @@ struct FrameParams {
     uint4 *big_rec;
     uint4 *ck;
     uint32_t ck_on;
-    uint32_t *cktab;
+    uint2 *cktab;
     uint32_t mstride;
-    uint32_t *ck_count, *ck_count_next;
+    uint2 *ck_count, *ck_count_next;  // per tile (list length, ~least occluder bound)
     uint32_t ck_zero_n;
     uint32_t tri_blocks, setup_blocks, span_parts;
     TileBin *count_next;

@@ -465,7 +465,7 @@ static_assert(kHash == 512u && kPairs <= (1u << 23), "pairs pack a 9-bit hash sl
 // ---------------------------------------------------------------------------
 // A span block of a checkpoint frame's setup launch (defined with the raster's
 // row helpers below).
-EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b);
+EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b, void *static_lds);
 
 // LDS of k_setup: the workgroup's records on their way out, then the binning tables.
 union SetupShared {
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     if constexpr (!BATCH) {
         if (blockIdx.x >= p.tri_blocks) {  // (checkpoint frames: tri_blocks = setup_blocks)
             if constexpr (SPAN) {
-                span_item(p, blockIdx.x - p.setup_blocks);
+                span_item(p, blockIdx.x - p.setup_blocks, &sm);
             } else if constexpr (vd_count(SH) > 0) {  // a vertex block
                 const View &v = p.v0;
                 const uint32_t t = (blockIdx.x - p.tri_blocks) * 256u + tid;
@@ -1064,7 +1064,9 @@ EUCHIP_DEV void raster_batch(const FrameParams &p, const View &v, TileShared &sh
 
 // The big triangles of `tile` from their row checkpoints: the tile's list of
 // (triangle, rows whose span reaches into the tile), 256 entries at a time
-// (one per lane, coalesced; the first 256 issued when the tile starts).  A
+// (one per lane, coalesced; the first 256 issued when the tile starts).  An
+// entry whose key-order bound lies above `occl`, the least bound of the tiles'
+// full-cover triangles, has no rows here (it loses every pixel).  A
 // prefix sum over the row counts places each entry's rows in a task table
 // (entry, row), and each lane takes table entries and walks one row from its
 // checkpoint (weights and the span inside the tile's column, at the slot of
@@ -1077,14 +1079,16 @@ struct BigTaskLds {
 static_assert(sizeof(BigTaskLds) <= sizeof(TriLds) * kBatch, "big-triangle tasks fit the staging space");
 
 EUCHIP_DEV void big_tasks(const FrameParams &p, TileShared &sh, uint32_t tid, uint32_t tile, uint32_t nbig,
-                          const TileGeom &g, uint32_t mode, uint32_t e_first) {
+                          uint32_t occl, const TileGeom &g, uint32_t mode, uint2 e_first) {
     BigTaskLds &bt = *reinterpret_cast<BigTaskLds *>(sh.tri);
-    const uint32_t *list = p.cktab + (size_t)tile * p.mstride;
+    const uint2 *list = p.cktab + (size_t)tile * p.mstride;
     const uint4 *cktile = p.ck + (size_t)(g.X0 >> g.sx) * p.v0.H + g.Y0;  // triangle t's at + t * ntx * H
     const size_t ckstride = (size_t)p.v0.ntx * p.v0.H;
     for (uint32_t s0 = 0; s0 < nbig; s0 += 256u) {
-        const uint32_t e = s0 ? (s0 + tid < nbig ? list[s0 + tid] : 0u) : e_first;
-        const uint32_t m = s0 + tid < nbig ? e & 0xffu : 0u;
+        const uint2 e2 = s0 ? (s0 + tid < nbig ? list[s0 + tid] : make_uint2(0u, 0u)) : e_first;
+        const uint32_t e = e2.x;
+        // behind the tile's occluder at every pixel: no rows
+        const uint32_t m = s0 + tid < nbig && e2.y <= occl ? e & 0xffu : 0u;
         const uint32_t mine = __builtin_popcount(m);
         uint32_t at = block_inclusive_sum(sh, tid, mine) - mine;
         const uint32_t total = sh.wave_rows[0] + sh.wave_rows[1] + sh.wave_rows[2] + sh.wave_rows[3];
@@ -1184,7 +1188,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     }
     for (uint32_t i = blockIdx.x * 256u + tid; i < p.zero_n; i += gridDim.x * 256u) p.count_next[i] = TileBin{0u, 0u};
     if constexpr (BIG)  // after a checkpoint frame: its list counts
-        for (uint32_t i = blockIdx.x * 256u + tid; i < p.ck_zero_n; i += gridDim.x * 256u) p.ck_count_next[i] = 0u;
+        for (uint32_t i = blockIdx.x * 256u + tid; i < p.ck_zero_n; i += gridDim.x * 256u) p.ck_count_next[i] = make_uint2(0u, 0u);
     // The pool ran out in setup: entries were lost, the frame is dropped (the
     // host sees the flag and re-runs it with a larger pool).
     if (flags & FLAG_BIN_OVERFLOW) return;
@@ -1234,9 +1238,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         const uint32_t *bin = p.bins + v.bin_base + (size_t)lt * cap;
         // the tile's checkpoint list: its length and first 256 entries, loaded
         // now (used after the bin; entries past the length are ignored)
-        uint32_t nck = 0u, e_first = 0u;
+        uint32_t nck = 0u, occl = 0u;
+        uint2 e_first = make_uint2(0u, 0u);
         if (BIG && ckp) {
-            nck = p.ck_count[tile];
+            const uint2 cc = p.ck_count[tile];
+            nck = cc.x, occl = ~cc.y;
             if (tid < p.mstride) e_first = p.cktab[(size_t)tile * p.mstride + tid];
         }
         __syncthreads();
@@ -1279,7 +1285,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         }
         if (BIG && ckp) {
             nck = __builtin_amdgcn_readfirstlane(nck);
-            if (nck) big_tasks(p, sh, tid, tile, nck, g, mode, e_first);
+            if (nck) big_tasks(p, sh, tid, tile, nck, __builtin_amdgcn_readfirstlane(occl), g, mode, e_first);
         }
         uint32_t ntile = 0u;
         TileBin ntb{0u, 0u};
@@ -1441,7 +1447,8 @@ static void launch_setup_t(const FrameParams &p, hipStream_t s, bool batched, ui
             const uint32_t sy = 8u - p.tile_sx;
             q.span_parts = (p.v0.H + (1u << sy) + kSpanRows - 1u) / kSpanRows;
             span_blocks = p.v0.n_tris * ((q.span_parts + 3u) / 4u);
-            lds = 4u * 4u * (((kSpanRows >> sy) * p.v0.ntx + 3u) / 4u);
+            const size_t tables = 4u * (size_t)span_table_bytes(sy, p.v0.ntx);  // in the static LDS when it fits
+            lds = tables <= sizeof(SetupShared) ? 0u : tables;
         }
         const size_t nv = (size_t)p.v0.n_instances * p.v0.n_vertices;
         if (span_blocks) {
@@ -1527,18 +1534,84 @@ static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, u
 // ---------------------------------------------------------------------------
 // Span blocks of a checkpoint frame's setup launch: block b walks rows of
 // triangle t = b % n_tris, its wave w the item 4 * (b / n_tris) + w (kSpanRows
-// rows from the triangle's first tile row).  The wave sets its triangle up
-// itself (tri_setup, as the setup blocks do) and leaves it to them unless its
+// rows from the triangle's first tile row).  Wave 0 sets the triangle up
+// (tri_setup, as the setup blocks do); the block leaves it to them unless its
 // box covers more than kBigTiles tiles.  A lane takes a row: the row span
 // (triangles.rs:219-254, with the row's band), a bit in the wave's LDS mask
 // table for every tile column the span touches, then the walk from x0
 // (triangles.rs:257-301, the same w += dx additions as the raster's walk)
-// storing the weights where the span enters each column.  The wave then appends
-// the entries of its tiles to their lists.  Slots sit at fixed places (layout
-// in internal.hpp), so no block waits for another.
+// storing the weights where the span enters each column; a second table
+// keeps a bit per row whose span holds the whole column.  The wave then
+// appends the entries of its tiles to their lists, with their key-order
+// bounds, and raises the occluder bound of each tile the triangle covers.
+// Slots sit at fixed places (layout in internal.hpp), so no block waits for
+// another.
 // ---------------------------------------------------------------------------
-EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b) {
-    extern __shared__ uint32_t span_lds[];  // per wave: (kSpanRows >> sy) x ntx mask bytes
+
+// Key-order bounds of a big triangle over one tile, for the occlusion test
+// (write modes).  Every weight the raster computes in the tile comes from the
+// walk of its row from x0 (triangles.rs:257-301): the start value and each of
+// at most (xmax - xmin) additions round by half an ulp of a partial sum, which
+// M bounds (|o| + |dy| y + |dx| x over the triangle's box), so each weight
+// stays within err = (xmax - xmin + 8) 2^-23 M of its plane, u2 within the sum
+// of those and its two subtractions, and z (three products, two sums) within
+// ez of the plane hz . (w0, w1, u2).  The planes are affine, so their extremes
+// over the tile's pixels lie at its corners, evaluated in f32 with the
+// evaluation's own rounding added to the margins (all terms doubled).
+struct TileBounds {
+    uint32_t klo;  // key order no fragment of the triangle in the tile goes below
+    uint32_t khi;  // key order none goes above (valid where `full`)
+    bool full;     // every pixel of the tile is a fragment of the triangle
+};
+
+EUCHIP_DEV TileBounds tile_bounds(const FrameParams &p, const TriRec &R, uint32_t mode, uint32_t X0, uint32_t Y0,
+                                  uint32_t TW, uint32_t TH) {
+    const float Xm = (float)R.xmax + 32.0f, Ym = (float)R.ymax;
+    float M[3], e[3];
+    const float K = (float)(R.xmax - R.xmin + 8u) * 0x1p-22f;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        M[i] = (fabsf(R.o[i]) + fabsf(R.dy[i]) * Ym + fabsf(R.dx[i]) * Xm) * 1.001f;
+        e[i] = K * M[i] + 0x1p-125f;
+    }
+    const float S = M[0] + M[1] + M[2];
+    const float eu = (e[0] + e[1] + e[2]) + 0x1p-21f * S;
+    const float h0 = fabsf(R.hz[0]), h1 = fabsf(R.hz[1]), h2 = fabsf(R.hz[2]);
+    const float P = h0 * M[0] + h1 * M[1] + h2 * S;
+    const float ez = 2.0f * ((h0 * e[0] + h1 * e[1] + h2 * eu) + 0x1p-18f * P) + 0x1p-124f;
+    // the corners' weights and z (planes of the reference's setup values)
+    const float xs[2] = {(float)X0, (float)(X0 + TW - 1u)}, ys[2] = {(float)Y0, (float)(Y0 + TH - 1u)};
+    float zmin = INFINITY, zmax = -INFINITY, wmin[3] = {INFINITY, INFINITY, INFINITY};
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            float w[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) w[i] = (R.o[i] + R.dy[i] * ys[c]) + R.dx[i] * xs[a];
+            const float u2 = (w[2] - w[0]) - w[1];
+            const float z = (R.hz[0] * w[0] + R.hz[1] * w[1]) + R.hz[2] * u2;
+            zmin = fminf(zmin, z), zmax = fmaxf(zmax, z);
+            wmin[0] = fminf(wmin[0], w[0]), wmin[1] = fminf(wmin[1], w[1]), wmin[2] = fminf(wmin[2], u2);
+        }
+    TileBounds b;
+    const bool finite = (zmin - zmin == 0.0f) & (zmax - zmax == 0.0f) & (ez - ez == 0.0f);
+    const float zlo = zmin - ez, zhi = zmax + ez;
+    // order values, as the walk's keys form them (-0 and +0 alike)
+    auto ord = [](float z) {
+        const uint32_t u = __float_as_uint(z + 0.0f);
+        return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
+    };
+    b.klo = !finite ? 0u : mode == 1u ? ord(zlo) : ~ord(zhi);
+    b.khi = mode == 1u ? ord(zhi) : ~ord(zlo);
+    const bool clip = !(R.flags & TRI_ALL_IN) && p.z_clip;
+    b.full = finite & (wmin[0] > e[0] + 0x1p-21f * M[0]) & (wmin[1] > e[1] + 0x1p-21f * M[1]) &
+             (wmin[2] > eu + 0x1p-20f * S) & (!clip | ((p.z_min <= zlo) & (zhi <= p.z_max)));
+    return b;
+}
+
+EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b, void *static_lds) {
+    extern __shared__ uint32_t span_lds[];  // the span tables when the static LDS is too small
     __shared__ TriRec span_tri;              // the block's triangle, set up once by wave 0
     __shared__ uint32_t span_keep;
     const View &v = p.v0;
@@ -1552,7 +1625,7 @@ EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b) {
     __syncthreads();
     if (!span_keep || part >= p.span_parts) return;
     const TriRec R = span_tri;
-    const uint32_t sx = p.tile_sx, sy = 8u - sx, TH = 1u << sy;
+    const uint32_t sx = p.tile_sx, sy = 8u - sx, TH = 1u << sy, TW = 1u << sx;
     const uint32_t col0 = R.xmin >> sx;
     const uint32_t cols = ((R.xmax - 1u) >> sx) - col0 + 1u;
     if (cols * (((R.ymax - 1u) >> sy) - (R.ymin >> sy) + 1u) <= kBigTiles) return;  // binned
@@ -1564,10 +1637,16 @@ EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b) {
     }
     const uint32_t ybase = ((R.ymin >> sy) << sy) + part * kSpanRows;
     if (ybase >= R.ymax) return;
-    const uint32_t groups = kSpanRows >> sy;
-    uint32_t *mw = span_lds + wv * ((groups * v.ntx + 3u) >> 2);
-    const uint8_t *mb = reinterpret_cast<const uint8_t *>(mw);
-    for (uint32_t i = lane; i < ((groups * cols + 3u) >> 2); i += 64u) mw[i] = 0u;
+    const uint32_t groups = kSpanRows >> sy, ne = groups * cols;
+    // this wave's tables of (tile row, column) bytes: rows reaching into the
+    // tile, rows whose span holds all of its columns
+    const uint32_t tb = span_table_bytes(sy, v.ntx);
+    uint8_t *base = 4u * tb <= sizeof(SetupShared) ? (uint8_t *)static_lds : (uint8_t *)span_lds;
+    uint8_t *const tab = base + wv * tb;
+    const uint32_t nb = (ne + 3u) & ~3u;
+    uint32_t *mw = (uint32_t *)tab, *fw = (uint32_t *)(tab + nb);
+    const uint8_t *mb = tab, *fb = tab + nb;
+    for (uint32_t i = lane; i < (nb >> 2); i += 64u) mw[i] = 0u, fw[i] = 0u;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const uint32_t y = ybase + lane;
@@ -1580,8 +1659,9 @@ EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b) {
     if (x0 < x1) {
         const uint32_t at0 = (lane >> sy) * cols - col0, bit = 1u << (y & (TH - 1u));
         for (uint32_t c = x0 >> sx; c <= ((x1 - 1u) >> sx); c++) {
-            const uint32_t at = at0 + c;
-            atomicOr(&mw[at >> 2], bit << ((at & 3u) * 8u));
+            const uint32_t at = at0 + c, sh = (at & 3u) * 8u;
+            atomicOr(&mw[at >> 2], bit << sh);
+            if (x0 <= (c << sx) && (c << sx) + TW <= x1) atomicOr(&fw[at >> 2], bit << sh);
         }
         const float Y = (float)y, X0 = (float)x0;
         float w0 = (R.o[0] + R.dy[0] * Y) + R.dx[0] * X0;
@@ -1590,7 +1670,7 @@ EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b) {
         uint4 *row = p.ck + (size_t)t * v.ntx * v.H + y;  // column c's slot at row + c * H
         for (uint32_t x = x0;;) {
             const uint32_t c = x >> sx, xc = c << sx;
-            const uint32_t xe = xc + (1u << sx) < x1 ? xc + (1u << sx) : x1;
+            const uint32_t xe = xc + TW < x1 ? xc + TW : x1;
             row[(size_t)c * v.H] = make_uint4(__float_as_uint(w0), __float_as_uint(w1), __float_as_uint(w2),
                                               (x - xc) | ((xe - xc) << 16));
             if (xe >= x1) break;
@@ -1612,14 +1692,23 @@ EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b) {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
     // the entries of this item's tiles (every tile row of it is this item's)
-    const uint32_t ty0 = ybase >> sy;
-    for (uint32_t i = lane; i < groups * cols; i += 64u) {
+    // with their bounds; a tile whose every row's span holds it and whose
+    // pixels the triangle covers (tile_bounds) raises its occluder bound.
+    // Key order: Less + write takes zord, Greater + write ~zord; other depth
+    // modes keep every entry (klo 0) and record no occluder.
+    const uint32_t mode = (p.depth_write && p.depth_test == DT_LESS) ? 1u
+                        : (p.depth_write && p.depth_test == DT_GREATER) ? 2u : 0u;
+    const uint32_t ty0 = ybase >> sy, all_rows = (1u << TH) - 1u;
+    for (uint32_t i = lane; i < ne; i += 64u) {
         const uint32_t m = mb[i];
         if (m == 0u) continue;
         const uint32_t gi = i / cols, c = i - gi * cols;
         const size_t tile = (size_t)(ty0 + gi) * v.ntx + col0 + c;
-        const uint32_t k = atomicAdd(&p.ck_count[tile], 1u);  // below mstride: one entry per triangle
-        p.cktab[tile * p.mstride + k] = (t << 8) | m;
+        TileBounds tb{0u, 0u, false};
+        if (mode != 0u) tb = tile_bounds(p, R, mode, (col0 + c) << sx, (ty0 + gi) << sy, TW, TH);
+        const uint32_t k = atomicAdd(&p.ck_count[tile].x, 1u);  // below mstride: one entry per triangle
+        p.cktab[tile * p.mstride + k] = make_uint2((t << 8) | m, tb.klo);
+        if (tb.full && fb[i] == all_rows) atomicMax(&p.ck_count[tile].y, ~tb.khi);
     }
 }
 

@@ -70,8 +70,8 @@ struct euchip_ctx {
     DevArray<uint32_t> big;     // big-triangle list
     DevArray<uint4> big_rec;    // per triangle of a checkpoint frame: what its row tasks read
     DevArray<uint4> ck;         // row checkpoints of big triangles
-    DevArray<uint32_t> cktab;   // per tile: its list of (triangle, row mask)
-    DevArray<uint32_t> ck_count;  // the lists' lengths, two halves like tile_count
+    DevArray<uint2> cktab;     // per tile: its list of (triangle, row mask; key-order bound)
+    DevArray<uint2> ck_count;  // the lists' lengths and occluder bounds, two halves like tile_count
     uint32_t ck_dirty[2] = {0, 0};  // counts a checkpoint frame left in each half, zeroed by the next frame
     DevArray<uint32_t> lrec;    // line records
     DevArray<uint64_t> lkey;    // per-pixel winner keys of a lines frame
@@ -572,7 +572,7 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         HIP_TRY(ctx, hipMemsetAsync(ctx->tile_count.ptr, 0, ctx->tile_count.cap * sizeof(TileBin), ctx->stream));
         ctx->counts_used[0] = ctx->counts_used[1] = 0;
         if (ctx->ck_count.cap)
-            HIP_TRY(ctx, hipMemsetAsync(ctx->ck_count.ptr, 0, ctx->ck_count.cap * 4u, ctx->stream));
+            HIP_TRY(ctx, hipMemsetAsync(ctx->ck_count.ptr, 0, ctx->ck_count.cap * sizeof(uint2), ctx->stream));
         ctx->ck_dirty[0] = ctx->ck_dirty[1] = 0;
     }
     const size_t half = ctx->tile_count.cap / 2;
@@ -584,20 +584,20 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
     // cannot hold leaves the big triangles on the big list, walked from the row
     // starts.  Tiles at most 8 rows high (an 8-bit row mask per tile and
     // triangle), at most 64 MB of tile entries and 1 GiB of checkpoints (one
-    // slot per triangle, tile column and row), and the span blocks' LDS mask
-    // tables (4 waves x 64 rows x tile columns bytes) within 64 KB.
+    // slot per triangle, tile column and row), and the span blocks' LDS
+    // tables (4 waves, span_table_bytes each) within 64 KB.
     bool ck_on = false;
     const size_t mstride = ((size_t)P.n_tris + 3) & ~(size_t)3;
     const size_t ck_slots = (size_t)P.n_tris * p.v0.ntx * p.v0.H;
     if (big_mode && p.tile_sx >= 5 && ntiles * mstride <= ((size_t)8 << 20) && ck_slots <= kCkMaxSlots &&
-        4u * (kSpanRows >> (8u - p.tile_sx)) * (size_t)p.v0.ntx <= ((size_t)64 << 10)) {
+        4u * (size_t)span_table_bytes(8u - p.tile_sx, p.v0.ntx) <= ((size_t)64 << 10)) {
         const size_t counts_had = ctx->ck_count.cap;
         ck_on = ensure(ctx, ctx->big_rec, (size_t)P.n_tris * kBigRecQuads, "big triangles", ctx->stream) == EUCHIP_OK &&
                 ensure(ctx, ctx->ck, ck_slots, "row checkpoints", ctx->stream) == EUCHIP_OK &&
                 ensure(ctx, ctx->cktab, ntiles * mstride, "checkpoint lists", ctx->stream) == EUCHIP_OK &&
                 ensure(ctx, ctx->ck_count, ctx->tile_count.cap, "checkpoint lists", ctx->stream) == EUCHIP_OK;
         if (ck_on && ctx->ck_count.cap != counts_had) {  // both halves start at zero
-            HIP_TRY(ctx, hipMemsetAsync(ctx->ck_count.ptr, 0, ctx->ck_count.cap * 4u, ctx->stream));
+            HIP_TRY(ctx, hipMemsetAsync(ctx->ck_count.ptr, 0, ctx->ck_count.cap * sizeof(uint2), ctx->stream));
             ctx->ck_dirty[0] = ctx->ck_dirty[1] = 0;
         }
     }
@@ -622,7 +622,8 @@ euchip_status render_impl(euchip_ctx *ctx, const euchip_pipeline_desc *d, const 
         // cleared here first.
         if (ck_on) {
             p.ck_count = ctx->ck_count.ptr + par * half;
-            if (ctx->ck_dirty[par]) HIP_TRY(ctx, launch_fill_u32(p.ck_count, 0u, ctx->ck_dirty[par], ctx->stream));
+            if (ctx->ck_dirty[par])
+                HIP_TRY(ctx, launch_fill_u32((uint32_t *)p.ck_count, 0u, 2u * (size_t)ctx->ck_dirty[par], ctx->stream));
             ctx->ck_dirty[par] = 0;
         }
         p.ck_count_next = nullptr, p.ck_zero_n = 0u;

@@ -10,7 +10,11 @@ other paths that mode has:
   * every tile shape the context takes for big-triangle frames: 64x4, 128x2 and
     256x1 tiles use the checkpoints with 4-, 2- and 1-row masks, 8x32 and 16x16
     tiles (taller than 8 rows) walk from the row starts;
-  * every depth mode, both band policies and MSAA through the checkpoint path.
+  * every depth mode, both band policies and MSAA through the checkpoint path;
+  * occlusion: entries behind a tile's full-cover triangle are skipped, so
+    depths within a few ulps of each other (and exact ties), depths that leave
+    the clip range and vertices far outside the target (wide margins) must
+    still give the oracle's pixels.
 """
 import os
 
@@ -172,3 +176,49 @@ def test_checkpoint_frames_among_other_frames():
             assert_bit_exact(r, f"frame {i} ({kind})")
     finally:
         ctx.close()
+
+
+def tie_triangles(seed, n, z0=0.5, spread=4, slope=True):
+    """n corner triangles (boxes over most of the target) whose depths lie within
+    `spread` ulps of z0 at every vertex: planes that tie exactly, or cross one
+    another by a few ulps, so that the winners of neighbouring pixels differ and
+    every occlusion bound sits next to the depths it has to separate."""
+    rng = np.random.default_rng(seed)
+    m = corner_triangles(seed, n)
+    pos = m.pos.copy()
+    base = np.float32(z0)
+    steps = rng.integers(0, spread + 1, (n, 1 if not slope else 3))
+    z = np.full((n, 3), base, np.float32)
+    for k in range(spread):
+        z = np.where(np.broadcast_to(steps, (n, 3)) > k, np.nextafter(z, np.float32(2.0)), z)
+    z[rng.random(n) < 0.25] = base  # exact ties (order breaks them)
+    pos[:, 2] = z.reshape(-1)
+    return S.Mesh(pos=pos, count=m.count, rgba8=m.rgba8)
+
+
+@pytest.mark.parametrize("dm", [DepthMode.LESS_WRITE, DepthMode.GREATER_WRITE])
+@pytest.mark.parametrize("slope", [False, True])
+def test_occlusion_bounds_near_ties(gpu_ctx, dm, slope):
+    # Entries behind a tile's full-cover triangle are skipped in the raster; the
+    # bounds carry margins for the walk's roundings, so triangles within a few
+    # ulps of the occluder (or tied with it) must all stay
+    mesh = tie_triangles(31 + slope, 48, slope=slope)
+    init = 0.0 if dm.test == 3 else 1.0
+    r = run_pair(gpu_ctx, flat(dm=dm), S.uniforms(), mesh, 700, 500, depth_init=init)
+    assert_bit_exact(r, f"ulp-close corner triangles {dm} slope={slope}")
+    assert r["frags"] > 5 * 10 ** 5
+
+
+def test_occlusion_bounds_clip_and_large_weights(gpu_ctx):
+    # occluders whose depth leaves [0, 1] somewhere (z clip) and triangles with
+    # vertices far outside the target (large weights, wide margins)
+    rng = np.random.default_rng(41)
+    m = corner_triangles(41, 40)
+    pos = m.pos.copy()
+    pos[:, 2] = rng.uniform(-0.4, 1.4, len(pos)).astype(np.float32)
+    far = rng.random(40) < 0.5
+    pos.reshape(40, 3, 4)[far, :, :2] *= np.float32(40.0)
+    mesh = S.Mesh(pos=pos, count=m.count, rgba8=m.rgba8)
+    r = run_pair(gpu_ctx, flat(), S.uniforms(), mesh, 640, 480)
+    assert r["frags"] > 10 ** 5
+    assert_bit_exact(r, "clipped and far corner triangles")
