@@ -95,12 +95,12 @@ constexpr uint32_t kHeavyTile = 128u;
 enum BusyMap : uint32_t { BUSY_RANGES = 0, BUSY_BLOCKS = 1 };
 
 // One-request frames of meshes with no more triangles than the target has tiles
-// ("big-triangle mode"): a triangle whose box covers more tiles than this is
-// not binned (binning it costs one global atomic per tile when the
-// workgroup's LDS grouping overflows).  With row checkpoints (below) every
-// tile scans its checkpoint entries of all triangles; without them the big
-// triangles go to the big-triangle list (while it has room) and every tile
-// walks them from the row starts.
+// ("big-triangle mode").  With row checkpoints (below) no triangle is binned:
+// the span blocks walk them all and every tile reads its checkpoint list.
+// Without them a triangle whose box covers more tiles than kBigTiles is not
+// binned (binning it costs one global atomic per tile when the workgroup's LDS
+// grouping overflows) but goes to the big-triangle list (while it has room),
+// and every tile walks those from the row starts.
 constexpr uint32_t kBigTiles = 64;
 constexpr uint32_t kBigCap = 4096;
 

@@ -526,15 +526,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
             ty0 = R.ymin >> sy;
             nx = ((R.xmax - 1u) >> sx) - tx0 + 1u;
             ny = ((R.ymax - 1u) >> sy) - ty0 + 1u;
-            if ((p.ck_on || p.big_cap) && nx * ny > kBigTiles) {  // not binned
-                if (p.ck_on) {
-                    nx = ny = 0u;  // the span blocks walk it
-                } else {  // on the big-triangle list (while it has room)
-                    const uint32_t k = atomicAdd(&p.ctr->big[p.frame & 1u][0], 1u);
-                    if (k < p.big_cap) {
-                        p.big[k] = t;
-                        nx = ny = 0u;
-                    }
+            if (p.ck_on) {
+                // checkpoint frames: the span blocks walk every triangle (a lane
+                // binning a triangle of many tiles here would take them one by
+                // one, each behind dependent LDS atomics)
+                nx = ny = 0u;
+            } else if (p.big_cap && nx * ny > kBigTiles) {  // on the big-triangle list (while it has room)
+                const uint32_t k = atomicAdd(&p.ctr->big[p.frame & 1u][0], 1u);
+                if (k < p.big_cap) {
+                    p.big[k] = t;
+                    nx = ny = 0u;
                 }
             }
             uint4 q0[kRec0Quads], q1[kRec1Quads];
@@ -1535,8 +1536,8 @@ static void launch_raster_t(const FrameParams &p, hipStream_t s, bool batched, u
 // Span blocks of a checkpoint frame's setup launch: block b walks rows of
 // triangle t = b % n_tris, its wave w the item 4 * (b / n_tris) + w (kSpanRows
 // rows from the triangle's first tile row).  Wave 0 sets the triangle up
-// (tri_setup, as the setup blocks do); the block leaves it to them unless its
-// box covers more than kBigTiles tiles.  A lane takes a row: the row span
+// (tri_setup, as the setup blocks do, which write its records and bin none of
+// a checkpoint frame's triangles).  A lane takes a row: the row span
 // (triangles.rs:219-254, with the row's band), a bit in the wave's LDS mask
 // table for every tile column the span touches, then the walk from x0
 // (triangles.rs:257-301, the same w += dx additions as the raster's walk)
@@ -1628,7 +1629,6 @@ EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b, void *static_lds) {
     const uint32_t sx = p.tile_sx, sy = 8u - sx, TH = 1u << sy, TW = 1u << sx;
     const uint32_t col0 = R.xmin >> sx;
     const uint32_t cols = ((R.xmax - 1u) >> sx) - col0 + 1u;
-    if (cols * (((R.ymax - 1u) >> sy) - (R.ymin >> sy) + 1u) <= kBigTiles) return;  // binned
     if (part == 0u && lane == 0u) {  // what the raster's row tasks read besides the slot
         uint4 *dst = p.big_rec + (size_t)t * kBigRecQuads;
         dst[0] = make_uint4(t, R.flags << 30, __float_as_uint(R.dx[0]), __float_as_uint(R.dx[1]));
