@@ -10,12 +10,12 @@
 //             grouped by tile in an LDS hash so each distinct tile costs one
 //             global atomic per workgroup; ranks past a tile's bin capacity go
 //             to overflow segments of that tile taken from a pool.  With few
-//             triangles per tile (big-triangle mode) triangles covering many
-//             tiles are not binned: span blocks of the same launch set each
-//             one up again, walk its rows once and store the row's weights
-//             where it enters each tile column, plus per (tile, triangle) the
-//             rows that reach into the tile (or, without checkpoints, the
-//             setup blocks put it on one list).
+//             triangles per tile (big-triangle mode) and row checkpoints no
+//             triangle is binned: span blocks of the same launch set each one
+//             up again, walk its rows once and store the row's weights where
+//             it enters each tile column, plus per (tile, triangle) the rows
+//             that reach into the tile and an occlusion bound (without
+//             checkpoints, triangles covering many tiles go on one list).
 //   k_raster  256-thread workgroups that take tiles (256 pixels, one per
 //             thread; 32x8, 16x16 or 8x32) from per-XCD queues.  Each pixel
 //             keeps a 64-bit key in LDS that encodes the winning fragment; the
@@ -956,6 +956,7 @@ EUCHIP_DEV void raster_row(const FrameParams &p, const TriRec &R, uint32_t prim,
         w2 = w2 + R.dx[2];
     }
     const uint32_t pi0 = r * (g.TW + 1u) + (xs - g.X0);
+    // the long-row variant (big-triangle kernels) walks two pixels per turn
     if (mode == 1u) walk_span<1u, (SKIP_UNROLL > 4u)>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
     else if (mode == 2u) walk_span<2u, (SKIP_UNROLL > 4u)>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
     else walk_row<0u>(p, R, prim, xs, xe, pi0, w0, w1, w2, key, d0s);
