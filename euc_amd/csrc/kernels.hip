@@ -1675,9 +1675,9 @@ EUCHIP_DEV void span_item(const FrameParams &p, uint32_t b, void *static_lds) {
             row[(size_t)c * v.H] = make_uint4(__float_as_uint(w0), __float_as_uint(w1), __float_as_uint(w2),
                                               (x - xc) | ((xe - xc) << 16));
             if (xe >= x1) break;
-            for (; x + 8u <= xe; x += 8u) {
+            for (; x + 16u <= xe; x += 16u) {
 #pragma unroll
-                for (uint32_t q = 0; q < 8u; q++) {
+                for (uint32_t q = 0; q < 16u; q++) {
                     w0 = w0 + R.dx[0];
                     w1 = w1 + R.dx[1];
                     w2 = w2 + R.dx[2];
