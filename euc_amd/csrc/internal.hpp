@@ -56,6 +56,7 @@ enum FrameFlags : uint32_t { FLAG_BIN_OVERFLOW = 1u };
 struct Counters {
     uint32_t pool[2][32];     // words of the overflow pool taken
     uint32_t next[2][8][32];  // per XCD: next tile of the dynamic tile loop
+    uint32_t empty[2][8][32]; // per XCD: next group of tiles the empty-tile clears look at
     uint32_t heavy[2][32];    // entries of the heavy-tile list
     uint32_t big[2][32];      // entries of the big-triangle list
     uint32_t busy[2][8][32];  // per XCD: entries of the busy-tile list

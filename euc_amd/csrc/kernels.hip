@@ -1186,6 +1186,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
     }
     if (blockIdx.x < 8u && tid == 0u) {
         p.ctr->next[npar][xcd][0] = 0u;
+        p.ctr->empty[npar][xcd][0] = 0u;
         p.ctr->busy[npar][xcd][0] = 0u;
     }
     for (uint32_t i = blockIdx.x * 256u + tid; i < p.zero_n; i += gridDim.x * 256u) p.count_next[i] = TileBin{0u, 0u};
@@ -1394,22 +1395,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
         k = __builtin_amdgcn_readfirstlane(sh.next);
         __syncthreads();
     }
-    // 3. Empty tiles of this workgroup's share (every per_xcd-th tile of the
-    //    XCD's contiguous range): the clears, stores only, no LDS.
+    // 3. Empty tiles: groups of 8 consecutive tiles of the XCD's contiguous
+    //    range taken from its counter (the workgroups that finish their busy
+    //    tiles first take more of them), the clears, stores only.
     if ((BIG && nbig) || (!BATCH && !p.v0.clear_color_en && !p.v0.clear_depth_en)) return;
     uint32_t t0, t1;
     xcd_range(ntiles, xcd, t0, t1);
-    for (uint32_t base = t0 + w; base < t1; base += 8u * per_xcd) {
+    uint32_t *empty = &p.ctr->empty[par][xcd][0];
+    for (;;) {
+        if (tid == 0u) sh.next = atomicAdd(empty, 8u);
+        __syncthreads();
+        const uint32_t base = t0 + __builtin_amdgcn_readfirstlane(sh.next);
+        __syncthreads();  // sh.next is written again next turn
+        if (base >= t1) break;
         uint32_t c[8];
 #pragma unroll
         for (uint32_t j = 0; j < 8u; j++) {
-            const uint32_t t = base + j * per_xcd;
+            const uint32_t t = base + j;
             c[j] = t < t1 ? p.tile_count[t].count : 1u;
         }
 #pragma unroll
         for (uint32_t j = 0; j < 8u; j++) {
             if (c[j] != 0u) continue;
-            const uint32_t tile = base + j * per_xcd;
+            const uint32_t tile = base + j;
             uint32_t r = 0;
             if constexpr (BATCH) r = __builtin_amdgcn_readfirstlane(p.tile_req[tile]);
             const View &v = view_of<BATCH>(p, r);
