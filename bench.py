@@ -65,6 +65,9 @@ def parse():
     ap.add_argument("--budget-gib", type=float, default=16.0, help="device-memory budget of the serving workload")
     ap.add_argument("--icons", type=int, default=4096, help="requests per batch of the icons workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--icon-tile-width", type=int, choices=(8, 16, 32), default=None,
+                    help="icons: this tile width instead of the autotuner's pick (counter runs, so that each "
+                         "counter pass measures the same kernel)")
     ap.add_argument("--no-plain-path", action="store_true",
                     help="icons: leave out the comparison with one render() per request (runs under a counter "
                          "profiler, which cannot follow thousands of queued launches)")
@@ -429,7 +432,11 @@ def bench_icons(args, ctx, cpu_seconds):
         d.clear(0.5)
     b = ctx.batch().set(desc, [Request(m, c, d, sc.uniforms) for sc, m, (c, d) in zip(scenes, meshes, tg)],
                         clear_color=ICON_CLEAR, clear_depth=1.0)
-    tile_w, tune_ms = b.autotune(reps=3)
+    if args.icon_tile_width:
+        b.set_tile_width(args.icon_tile_width)
+        tile_w, tune_ms = args.icon_tile_width, None
+    else:
+        tile_w, tune_ms = b.autotune(reps=3)
     for _ in range(max(args.warmup, 2)):
         b.run(graph=True)
     lat = []
