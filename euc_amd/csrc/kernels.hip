@@ -1422,7 +1422,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_wave
             if constexpr (BATCH) r = __builtin_amdgcn_readfirstlane(p.tile_req[tile]);
             const View &v = view_of<BATCH>(p, r);
             const uint32_t lt = tile - v.tile_base;
-            const uint32_t ty = lt / v.ntx, tx = lt - ty * v.ntx;
+            // one-sample kernels: the divisor made opaque here (an empty asm),
+            // so that its reciprocal is computed in this loop and not kept live
+            // through the whole kernel (where it took a register the shading needs)
+            uint32_t ntx = v.ntx;
+            if constexpr (!MS && !BIG) {
+                ntx = __builtin_amdgcn_readfirstlane(ntx);
+                asm volatile("" : "+s"(ntx));
+            }
+            const uint32_t ty = lt / ntx, tx = lt - ty * ntx;
             const uint32_t px = (tx << sx) + (tid & ((1u << sx) - 1u)), py = (ty << sy) + (tid >> sx);
             if (px < v.W && py < v.H) {
                 const size_t pidx = (size_t)py * v.W + px;
